@@ -1,0 +1,173 @@
+"""BM3D 1080p benchmark (single GPU).
+
+Workload: plain two-pass BM3Dv2 (basic hard-threshold pass, then Wiener with ref = basic),
+chroma=True on one 1920x1080 YUV444PS frame; sigma=[5,3,3], block_step=4, bm_range=16,
+ps_num=2, ps_range=4, ssd, dct/dct, extractor_exp=3, float32. Inputs are synthetic (seeded
+generator) and resident in HBM before the timed region. One step = both passes (matching,
+filtering, aggregation, division) over the frame. value = Mpix/s = 1920*1080 / step time.
+
+Roofline: the dominant kernel's algorithmic work is counted from the reference geometry:
+SSD_FLOP = 192 per candidate block evaluation, summed over the clipped windows of every
+reference block of both passes. Its duration comes from HIP events on the launch stream.
+CPU baseline: the FP32 oracle (multithreaded C++) on a bounded strip of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (REPO, os.path.join(REPO, "vapoursynth-zipcu_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "BM3D 1080p Mpix/s: plain FP32 vs learned+BF16 form at ≤0.05 dB PSNR loss"
+W, H = 1920, 1080
+SIGMA = [5.0, 3.0, 3.0]
+PARAMS = dict(block_step=4, bm_range=16, ps_num=2, ps_range=4, chroma=True, extractor_exp=3,
+              metric="ssd", t2d="dct", t1d="dct")
+PEAK_FP32_TFLOPS = 157.3
+
+
+def origins(n, step):
+    b = np.arange(0, (n - 8 + step - 1) // step + 1)
+    return np.minimum(b * step, n - 8)
+
+
+def ssd_candidates(width, height, step, rng):
+    """Candidate block evaluations of one spatial pass (clipped windows, src/bm3d.cu:618-660)."""
+    xs, ys = origins(width, step), origins(height, step)
+    nx = np.minimum(xs + rng, width - 8) - np.maximum(xs - rng, 0) + 1
+    ny = np.minimum(ys + rng, height - 8) - np.maximum(ys - rng, 0) + 1
+    return int(nx.sum()) * int(ny.sum())
+
+
+def cpu_baseline(nz, strip_rows):
+    import oracle
+    from vszipcu import device
+    threads = min(os.cpu_count() or 1, 64)
+    strip = np.ascontiguousarray(nz[:, :strip_rows])
+    sb = device.scaled_sigma(SIGMA, False)
+    sf = device.scaled_sigma(SIGMA, True)
+    common = {k: v for k, v in PARAMS.items()}
+    pb = oracle.make_params(W, strip_rows, sigma=tuple(sb), threads=threads, **common)
+    pf = oracle.make_params(W, strip_rows, sigma=tuple(sf), final=True, threads=threads, **common)
+    t0 = time.perf_counter()
+    res, _, _ = oracle.bm3d(pb, strip[None, :, None])
+    basic = oracle.aggregate(res[:, 0], W)
+    res2, _, _ = oracle.bm3d(pf, np.stack([basic, strip])[:, :, None])
+    oracle.aggregate(res2[:, 0], W)
+    dt = time.perf_counter() - t0
+    return {"value": round(W * strip_rows / dt / 1e6, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle, both passes on a {W}x{strip_rows} strip of the same frame "
+                      f"({dt:.1f} s wall, {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-strip-rows", type=int, default=272)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the BM3D path shards by frame or plane, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from vszipcu import device, synth
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    print(f"bench: workload 1920x1080 YUV444 two-pass BM3D, steps={args.steps} warmup={args.warmup}", file=sys.stderr,
+          flush=True)
+    clean, nz = synth.yuv444_frame(W, H, SIGMA, seed=2)
+    kp_b = device.kparams(W, H, sigma=device.scaled_sigma(SIGMA, False), **PARAMS)
+    kp_f = device.kparams(W, H, sigma=device.scaled_sigma(SIGMA, True), final=True, **PARAMS)
+    d_noisy = torch.from_numpy(nz[:, None]).cuda()            # [3, 1, H, W]
+    d_basic = torch.zeros((3, 1, H, W), dtype=torch.float32, device="cuda")
+    d_final = torch.zeros((3, H, W), dtype=torch.float32, device="cuda")
+    work_b = torch.zeros(device.workspace_size(kp_b), dtype=torch.uint8, device="cuda")
+    work_f = torch.zeros(device.workspace_size(kp_f), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        device.run(kp_b, d_noisy, None, work_b, d_basic.view(3, H, W), 0, stream)
+        device.run(kp_f, d_basic, d_noisy, work_f, d_final, 0, stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    mpix = W * H / dt / 1e6
+    print(f"bench: {dt * 1e3:.3f} ms/step, {mpix:.2f} Mpix/s", file=sys.stderr, flush=True)
+
+    # per-kernel timing with HIP events on the launch stream: the matching kernels of both
+    # passes (re-launched alone on the same inputs) and the remainder of each pass
+    nbx, nby = device.num_blocks(kp_b)
+    mb = work_b[: nbx * nby * 128].view(torch.int32).view(nby, nbx, 8, 4)
+    mf = work_f[: nbx * nby * 128].view(torch.int32).view(nby, nbx, 8, 4)
+    reps = max(3, args.steps // 2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        device.match(kp_b, d_noisy[0], mb, stream)
+    ev[1].record(stream)
+    for _ in range(reps):
+        device.match(kp_f, d_basic[0], mf, stream)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    t_match_b = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    t_match_f = ev[1].elapsed_time(ev[2]) / reps * 1e-3
+    cand = ssd_candidates(W, H, PARAMS["block_step"], PARAMS["bm_range"])
+    flop = 192.0 * cand * 2
+    achieved = flop / (t_match_b + t_match_f) / 1e12
+    t_match = t_match_b + t_match_f
+    print(f"bench: match kernels {t_match_b * 1e3:.3f} + {t_match_f * 1e3:.3f} ms, "
+          f"{cand} candidates/pass, {achieved:.2f} TFLOP/s", file=sys.stderr, flush=True)
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "match_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = d_final.cpu().numpy()
+    psnr_y = synth.psnr(out[0], clean[0])
+
+    result = {
+        "metric": METRIC,
+        "value": round(mpix, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "BM3Dv2 two-pass (basic + Wiener ref=basic), chroma, 1920x1080 YUV444PS",
+                   "sigma": SIGMA, "path": "plain FP32", **{k: v for k, v in PARAMS.items()},
+                   "psnr_y_db": round(psnr_y, 3), "match_ms_per_step": round(t_match * 1e3, 4)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "k_match (SSD block matching, both passes)"},
+    }
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(nz, args.cpu_strip_rows)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,777 @@
+// BM3D device path for gfx950 (CDNA4): block matching, collaborative filtering, exact
+// aggregation, final division and temporal aggregation, plus the kernel-layer C ABI
+// (vsz_bm3d_match / vsz_bm3d_run / vsz_vaggregate_run) declared in include/vszipcu_bm3d.h.
+//
+// Reference path: src/bm3d.cu:571-922 (kernels `bm3d`, `aggregate`) and src/bm3d_vagg.cu:18-48
+// (`vaggregate`), launched by src/bm3d.zig:375-401 and :589-630.
+//
+// Numerics contract (identical in the CPU oracle): built without FP contraction and without
+// fast math; every fused multiply-add is an explicit fmaf; divisions and sqrt are IEEE
+// correctly rounded; reductions follow the reference's per-lane order and the 8-lane
+// butterfly ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)). Aggregation with a non-zero extractor
+// E = 2^e sums the quantised contributions q(u) = (u+E)-E, which all lie on the grid
+// 2^(e-24), exactly in 64-bit integers (device-wide atomics commute on integers, so the
+// result is deterministic); each sum is rounded once to FP32 before the IEEE division.
+// With E = 0 the sums are FP32 atomics, order-dependent exactly as in the reference.
+
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "vszipcu_bm3d.h"
+#include "internal.h"
+
+namespace vszk {
+
+// ------------------------------------------------------------------------------------------------
+// 8-point transforms (src/bm3d.cu:32-280). Same op sequence as the reference codelets: the
+// FMA/FMS/FNMS macros become fused multiply-adds, everything else is one rounding per op.
+// ------------------------------------------------------------------------------------------------
+namespace kc {
+constexpr float tan_pi_8 = float(0.414213562373095048801688724209698078569671875);
+constexpr float two_cos_pi_8 = float(1.847759065022573512256366378793576573644833252);
+constexpr float tan_pi_16 = float(0.198912367379658006911597622644676228597850501);
+constexpr float two_cos_pi_16 = float(1.961570560806460898252364472268478073947867462);
+constexpr float sqrt2 = float(1.414213562373095048801688724209698078569671875);
+constexpr float tan_3pi_16 = float(0.668178637919298919997757686523080761552472251);
+constexpr float two_cos_3pi_16 = float(1.662939224605090474157576755235811513477121624);
+constexpr float inv_sqrt2 = float(0.707106781186547524400844362104849039284835938);
+constexpr float two_sqrt2 = float(2.828427124746190097603377448419396157139343751);
+constexpr float b_877 = float(0.877670597010003062405456290501163901200537360);
+constexpr float b_1797 = float(1.797135031972863413496886690073811797696338403);
+constexpr float b_2277 = float(2.277437593371384746027611934034934695963515886);
+constexpr float b_1609 = float(1.609389232649111887192845766718020518480884560);
+constexpr float b_334 = float(0.334024180361136429417383083658457088741315663);
+constexpr float b_1495 = float(1.495435764250674860795011090214036706658653686);
+constexpr float b_2058 = float(2.058234225009388964222454285384072231477027482);
+constexpr float b_486 = float(0.486135912065751423025580498947083714508324707);
+}  // namespace kc
+
+__device__ __forceinline__ float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ float fnms_(float a, float b, float c) { return __builtin_fmaf(-a, b, c); }
+__device__ __forceinline__ float fms_(float a, float b, float c) { return __builtin_fmaf(a, b, -c); }
+
+template <int KIND, bool FWD> __device__ __forceinline__ void xf8(float *v) {
+    using namespace kc;
+    if constexpr (KIND == 0 && FWD) {  // dct, forward
+        const float a07 = v[0] - v[7], s07 = v[0] + v[7];
+        const float a43 = v[4] - v[3], s43 = v[4] + v[3];
+        const float a25 = v[2] - v[5], a16 = v[1] - v[6];
+        const float p = a25 + a16, s16 = v[1] + v[6];
+        const float m = a25 - a16, s25 = v[2] + v[5];
+        const float tb = fnms_(inv_sqrt2, p, a07);
+        const float tg = fnms_(inv_sqrt2, m, a43);
+        const float o3 = two_cos_3pi_16 * fmaf_(tan_3pi_16, tg, tb);
+        const float o5 = -(two_cos_3pi_16 * fnms_(tan_3pi_16, tb, tg));
+        const float tp = s07 + s43, tq = s25 + s16;
+        const float o4 = sqrt2 * (tp - tq);
+        const float o0 = sqrt2 * (tp + tq);
+        const float th = fmaf_(inv_sqrt2, p, a07);
+        const float ti = fmaf_(inv_sqrt2, m, a43);
+        const float o1 = two_cos_pi_16 * fnms_(tan_pi_16, ti, th);
+        const float o7 = two_cos_pi_16 * fmaf_(tan_pi_16, th, ti);
+        const float tl = s07 - s43, to = s25 - s16;
+        const float o2 = two_cos_pi_8 * fnms_(tan_pi_8, to, tl);
+        const float o6 = two_cos_pi_8 * fmaf_(tan_pi_8, tl, to);
+        v[0] = o0; v[1] = o1; v[2] = o2; v[3] = o3; v[4] = o4; v[5] = o5; v[6] = o6; v[7] = o7;
+    } else if constexpr (KIND == 0) {  // dct, inverse
+        const float t1 = v[0] * sqrt2;
+        const float t3 = fmaf_(sqrt2, v[4], t1);
+        const float tj = fnms_(sqrt2, v[4], t1);
+        const float t6 = fmaf_(tan_pi_8, v[6], v[2]);
+        const float tk = fms_(tan_pi_8, v[2], v[6]);
+        const float tb = v[5] + v[3];
+        const float te = v[3] - v[5];
+        const float tc = fmaf_(inv_sqrt2, tb, v[1]);
+        const float tn = fnms_(inv_sqrt2, te, v[7]);
+        const float tf = fmaf_(inv_sqrt2, te, v[7]);
+        const float tm = fnms_(inv_sqrt2, tb, v[1]);
+        const float t7 = fmaf_(two_cos_pi_8, t6, t3);
+        const float tg = fmaf_(tan_pi_16, tf, tc);
+        const float o7 = fnms_(two_cos_pi_16, tg, t7);
+        const float o0 = fmaf_(two_cos_pi_16, tg, t7);
+        const float tp = fnms_(two_cos_pi_8, tk, tj);
+        const float tq = fmaf_(tan_3pi_16, tm, tn);
+        const float o5 = fnms_(two_cos_3pi_16, tq, tp);
+        const float o2 = fmaf_(two_cos_3pi_16, tq, tp);
+        const float th = fnms_(two_cos_pi_8, t6, t3);
+        const float ti = fnms_(tan_pi_16, tc, tf);
+        const float o3 = fnms_(two_cos_pi_16, ti, th);
+        const float o4 = fmaf_(two_cos_pi_16, ti, th);
+        const float tl = fmaf_(two_cos_pi_8, tk, tj);
+        const float to = fnms_(tan_3pi_16, tn, tm);
+        const float o6 = fnms_(two_cos_3pi_16, to, tl);
+        const float o1 = fmaf_(two_cos_3pi_16, to, tl);
+        v[0] = o0; v[1] = o1; v[2] = o2; v[3] = o3; v[4] = o4; v[5] = o5; v[6] = o6; v[7] = o7;
+    } else if constexpr (KIND == 1 && FWD) {  // haar, forward
+        const float s01 = v[0] + v[1], d01 = v[0] - v[1];
+        const float s23 = v[2] + v[3], d23 = v[2] - v[3];
+        const float s45 = v[4] + v[5], d45 = v[4] - v[5];
+        const float s67 = v[6] + v[7], d67 = v[6] - v[7];
+        const float a = s01 + s23, b = sqrt2 * (s01 - s23);
+        const float c = s45 + s67, d = sqrt2 * (s45 - s67);
+        const float sc2 = sqrt2 * 2.0f;
+        v[0] = sqrt2 * (a + c); v[1] = sqrt2 * (a - c); v[2] = sqrt2 * b; v[3] = sqrt2 * d;
+        v[4] = sc2 * d01; v[5] = sc2 * d23; v[6] = sc2 * d45; v[7] = sc2 * d67;
+    } else if constexpr (KIND == 1) {  // haar, inverse (rank 4 in the reference; kept literally)
+        const float s01 = v[0] + v[1], d01 = v[0] - v[1];
+        const float p2 = sqrt2 * v[2], q4 = 2.0f * v[4], n2 = (-sqrt2) * v[2];
+        const float pp = p2 + q4, pm = p2 - q4, np = n2 + q4, nm = n2 - q4;
+        v[0] = sqrt2 * (s01 + pp); v[1] = sqrt2 * (s01 + pm); v[2] = sqrt2 * (s01 + np); v[3] = sqrt2 * (s01 + nm);
+        v[4] = sqrt2 * (d01 + pp); v[5] = sqrt2 * (d01 + pm); v[6] = sqrt2 * (d01 + np); v[7] = sqrt2 * (d01 + nm);
+    } else if constexpr (KIND == 2) {  // wht (self-inverse up to 16)
+        const float a = v[0] + v[1], b = v[0] - v[1], c = v[2] + v[3], d = v[2] - v[3];
+        const float e = v[4] + v[5], f = v[4] - v[5], g = v[6] + v[7], h = v[6] - v[7];
+        const float ac = a + c, a_c = a - c, bd = b + d, b_d = b - d;
+        const float eg = e + g, e_g = e - g, fh = f + h, f_h = f - h;
+        v[0] = sqrt2 * (ac + eg); v[1] = sqrt2 * (ac - eg); v[2] = sqrt2 * (a_c - e_g); v[3] = sqrt2 * (a_c + e_g);
+        v[4] = sqrt2 * (b_d + f_h); v[5] = sqrt2 * (b_d - f_h); v[6] = sqrt2 * (bd - fh); v[7] = sqrt2 * (bd + fh);
+    } else if constexpr (FWD) {  // bior1.5 analysis
+        const float s03 = v[0] + v[3], d03 = v[0] - v[3], s12 = v[1] + v[2], d12 = v[1] - v[2];
+        const float s47 = v[4] + v[7], d47 = v[4] - v[7], s56 = v[5] + v[6], d56 = v[5] - v[6];
+        const float e01 = v[0] - v[1], e23 = v[2] - v[3], e45 = v[4] - v[5], e67 = v[6] - v[7];
+        const float o0 = sqrt2 * (((s03 + s47) + s12) + s56);
+        const float o1 = (b_877 * (s03 - s47)) + (b_1797 * (s12 - s56));
+        const float o2 = ((b_2277 * d03) + (b_1609 * d12)) + (b_334 * (d56 - d47));
+        const float o3 = ((b_2277 * d47) + (b_1609 * d56)) + (b_334 * (d12 - d03));
+        v[0] = o0; v[1] = o1; v[2] = o2; v[3] = o3;
+        v[4] = two_sqrt2 * e01; v[5] = two_sqrt2 * e23; v[6] = two_sqrt2 * e45; v[7] = two_sqrt2 * e67;
+    } else {  // bior1.5 synthesis
+        const float u0 = sqrt2 * v[0], u1 = b_1495 * v[1], u2 = b_2058 * v[2], u3 = b_2058 * v[3];
+        const float a4 = two_sqrt2 * v[4], b4 = b_486 * v[4], a5 = two_sqrt2 * v[5], b5 = b_486 * v[5];
+        const float a6 = two_sqrt2 * v[6], b6 = b_486 * v[6], a7 = two_sqrt2 * v[7], b7 = b_486 * v[7];
+        const float p = u0 + u1, q = u0 - u1, r = b5 - b7, s = b4 - b6;
+        v[0] = (p + u2) + (a4 - r); v[1] = (p + u2) - (a4 + r);
+        v[2] = (p - u2) + (s + a5); v[3] = (p - u2) + (s - a5);
+        v[4] = (q + u3) + (r + a6); v[5] = (q + u3) + (r - a6);
+        v[6] = (q - u3) - (s - a7); v[7] = (q - u3) - (s + a7);
+    }
+}
+
+__device__ __forceinline__ float tree8(const float *c) {
+    return ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
+}
+
+// ------------------------------------------------------------------------------------------------
+// Block distances (src/bm3d.cu:290-390). `a` is the reference patch (row-major 8x8 in
+// registers), `b` the candidate's top-left in the image.
+// ------------------------------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ float block_dist(const float *a, const float *__restrict__ b, int S) {
+    float col[8];
+    if constexpr (M == 0 || M == 1) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            float e0 = 0.0f, e1 = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 8; r += 2) {
+                const float d0 = a[r * 8 + c] - b[r * S + c];
+                const float d1 = a[(r + 1) * 8 + c] - b[(r + 1) * S + c];
+                if constexpr (M == 0) {
+                    e0 = fmaf_(d0, d0, e0);
+                    e1 = fmaf_(d1, d1, e1);
+                } else {
+                    e0 = e0 + fabsf(d0);
+                    e1 = e1 + fabsf(d1);
+                }
+            }
+            col[c] = e0 + e1;
+        }
+        return tree8(col);
+    } else if constexpr (M == 2 || M == 3) {
+        float ca[8], cb[8], bb[64];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) bb[r * 8 + c] = b[r * S + c];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            ca[c] = ((a[c] + a[8 + c]) + (a[16 + c] + a[24 + c])) + ((a[32 + c] + a[40 + c]) + (a[48 + c] + a[56 + c]));
+            cb[c] = ((bb[c] + bb[8 + c]) + (bb[16 + c] + bb[24 + c])) + ((bb[32 + c] + bb[40 + c]) + (bb[48 + c] + bb[56 + c]));
+        }
+        const float dm = tree8(ca) * (1.0f / 64.0f) - tree8(cb) * (1.0f / 64.0f);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            float e0 = 0.0f, e1 = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 8; r += 2) {
+                const float d0 = (a[r * 8 + c] - bb[r * 8 + c]) - dm;
+                const float d1 = (a[(r + 1) * 8 + c] - bb[(r + 1) * 8 + c]) - dm;
+                if constexpr (M == 2) {
+                    e0 = fmaf_(d0, d0, e0);
+                    e1 = fmaf_(d1, d1, e1);
+                } else {
+                    e0 = e0 + fabsf(d0);
+                    e1 = e1 + fabsf(d1);
+                }
+            }
+            col[c] = e0 + e1;
+        }
+        return tree8(col);
+    } else {  // ssd/norm
+        float qa[8], qb[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f, x0 = 0.0f, x1 = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 8; r += 2) {
+                const float va0 = a[r * 8 + c], vb0 = b[r * S + c];
+                const float va1 = a[(r + 1) * 8 + c], vb1 = b[(r + 1) * S + c];
+                a0 = fmaf_(va0, va0, a0);
+                b0 = fmaf_(vb0, vb0, b0);
+                x0 = fmaf_(va0, vb0, x0);
+                a1 = fmaf_(va1, va1, a1);
+                b1 = fmaf_(vb1, vb1, b1);
+                x1 = fmaf_(va1, vb1, x1);
+            }
+            qa[c] = a0 + a1;
+            qb[c] = b0 + b1;
+            col[c] = x0 + x1;
+        }
+        const float na = __fsqrt_rn(tree8(qa));
+        const float nb = __fsqrt_rn(tree8(qb));
+        return 2.0f - __fdiv_rn(2.0f * tree8(col), (na * nb) + FLT_EPSILON);
+    }
+}
+
+// Strict-< sorted insertion into an ascending 8-list (src/bm3d.cu:641-653).
+__device__ __forceinline__ void insert8(float (&er)[8], int (&xs)[8], int (&ys)[8], int (&zs)[8], float e, int x,
+                                        int y, int z) {
+    if (e < er[7]) {
+#pragma unroll
+        for (int k = 7; k > 0; --k) {
+            const bool fk = e < er[k];
+            const bool fp = e < er[k - 1];
+            if (fk) {
+                er[k] = fp ? er[k - 1] : e;
+                xs[k] = fp ? xs[k - 1] : x;
+                ys[k] = fp ? ys[k - 1] : y;
+                zs[k] = fp ? zs[k - 1] : z;
+            }
+        }
+        if (e < er[0]) {
+            er[0] = e;
+            xs[0] = x;
+            ys[0] = y;
+            zs[0] = z;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Matching: one lane per reference block (v1: direct loads, no LDS staging).
+// ------------------------------------------------------------------------------------------------
+template <int M, bool TEMPORAL>
+__global__ __launch_bounds__(256) void k_match(MatchArgs a) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= a.nbx * a.nby) return;
+    const int W = a.width, H = a.height, S = a.stride, R = TEMPORAL ? a.radius : 0;
+    const int x = min((gid % a.nbx) * a.block_step, W - 8);
+    const int y = min((gid / a.nbx) * a.block_step, H - 8);
+    const float *centre = a.img + (long long)R * a.ts;
+
+    float ref[64];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) ref[r * 8 + c] = centre[(long long)(y + r) * S + x + c];
+
+    float er[8];
+    int xs[8], ys[8], zs[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        er[k] = FLT_MAX;
+        xs[k] = 0;
+        ys[k] = 0;
+        zs[k] = R;
+    }
+
+    {
+        const int left = max(x - a.bm_range, 0), right = min(x + a.bm_range, W - 8);
+        const int top = max(y - a.bm_range, 0), bottom = min(y + a.bm_range, H - 8);
+        for (int yy = top; yy <= bottom; ++yy) {
+            const float *row = centre + (long long)yy * S;
+            for (int xx = left; xx <= right; ++xx) insert8(er, xs, ys, zs, block_dist<M>(ref, row + xx, S), xx, yy, R);
+        }
+    }
+
+    if constexpr (TEMPORAL) {
+        int cx[8], cy[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            cx[k] = xs[k];
+            cy[k] = ys[k];
+        }
+        for (int dir = -1; dir <= 1; dir += 2) {
+            int lx[8], ly[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                lx[k] = cx[k];
+                ly[k] = cy[k];
+            }
+            for (int t = 1; t <= R; ++t) {
+                const int tz = R + dir * t;
+                const float *frame = a.img + (long long)tz * a.ts;
+                float fe[8];
+                int fx[8], fy[8], fz[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    fe[k] = FLT_MAX;
+                    fx[k] = 0;
+                    fy[k] = 0;
+                    fz[k] = tz;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (i < a.ps_num) {
+                        const int left = max(lx[i] - a.ps_range, 0), right = min(lx[i] + a.ps_range, W - 8);
+                        const int top = max(ly[i] - a.ps_range, 0), bottom = min(ly[i] + a.ps_range, H - 8);
+                        for (int yy = top; yy <= bottom; ++yy) {
+                            const float *row = frame + (long long)yy * S;
+                            for (int xx = left; xx <= right; ++xx)
+                                insert8(fe, fx, fy, fz, block_dist<M>(ref, row + xx, S), xx, yy, tz);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (i < a.ps_num) insert8(er, xs, ys, zs, fe[i], fx[i], fy[i], tz);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    lx[k] = fx[k];
+                    ly[k] = fy[k];
+                }
+            }
+        }
+    }
+
+    // self-inclusion (src/bm3d.cu:758-788)
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) found |= (xs[k] == x && ys[k] == y && (!TEMPORAL || zs[k] == R));
+    if (!found) {
+#pragma unroll
+        for (int k = 7; k > 0; --k) {
+            er[k] = er[k - 1];
+            xs[k] = xs[k - 1];
+            ys[k] = ys[k - 1];
+            zs[k] = zs[k - 1];
+        }
+        er[0] = 0.0f;
+        xs[0] = x;
+        ys[0] = y;
+        zs[0] = R;
+    }
+    int4 *out = a.matches + (long long)gid * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = make_int4(xs[k], ys[k], zs[k], __float_as_int(er[k]));
+}
+
+// ------------------------------------------------------------------------------------------------
+// Collaborative filtering + aggregation: 8 lanes per group (lane = patch column), 8 groups
+// per 64-lane workgroup. Each lane holds v[i*8 + j] = value at (rank i, row j, its column).
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void transpose8x8(float *v, float *buf, int c) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) buf[j * 9 + c] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = buf[c * 9 + j];
+    __syncthreads();
+}
+
+template <int T2, int T1, bool FWD> __device__ __forceinline__ void xform3d(float *v, float *buf, int c) {
+    // 2-D: vertical pass, transpose, horizontal pass, transpose back (src/bm3d.cu:461-466).
+#pragma unroll
+    for (int i = 0; i < 8; ++i) xf8<T2, FWD>(v + i * 8);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) transpose8x8(v + i * 8, buf, c);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) xf8<T2, FWD>(v + i * 8);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) transpose8x8(v + i * 8, buf, c);
+    // 1-D along the group axis.
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float t[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t[i] = v[i * 8 + j];
+        xf8<T1, FWD>(t);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i * 8 + j] = t[i];
+    }
+}
+
+__device__ __forceinline__ float bfly8(float x) {
+    x += __shfl_xor(x, 1);
+    x += __shfl_xor(x, 2);
+    x += __shfl_xor(x, 4);
+    return x;
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float q, int e) {
+    float s = __builtin_ldexpf(q, 24 - e);
+    s = fminf(fmaxf(s, -4.611686018427387904e18f), 4.611686018427387904e18f);
+    return (unsigned long long)(long long)s;
+}
+
+template <bool FINAL, int T2, int T1>
+__global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
+    __shared__ float tbuf[8][72];
+    const int lane = threadIdx.x, g = lane >> 3, c = lane & 7;
+    const int gid = blockIdx.x * 8 + g;
+    const bool valid = gid < a.nblocks;
+    const int4 *ml = a.matches + (long long)(valid ? gid : 0) * 8;
+    int mx[8], my[8], mz[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int4 m = ml[k];
+        mx[k] = m.x;
+        my[k] = m.y;
+        mz[k] = m.z;
+    }
+    const long long TS = a.ts;
+    const int S = a.stride;
+    float *buf = tbuf[g];
+
+    for (int p = 0; p < a.num_planes; ++p) {
+        if (!((a.active_mask >> p) & 1)) continue;
+        const float sigma = a.sigma[p];
+        const float *pref = a.clip0 + (long long)p * a.plane_elems;
+        const float *pnoisy = FINAL ? a.clip1 + (long long)p * a.plane_elems : pref;
+        float v[64];
+        float r[FINAL ? 64 : 1];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const long long off = (long long)mz[i] * TS + (long long)my[i] * S + mx[i] + c;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                v[i * 8 + j] = pnoisy[off + (long long)j * S];
+                if constexpr (FINAL) r[i * 8 + j] = pref[off + (long long)j * S];
+            }
+        }
+        xform3d<T2, T1, true>(v, buf, c);
+        float k = 0.0f;
+        if constexpr (FINAL) {
+            xform3d<T2, T1, true>(r, buf, c);
+            const float s2 = sigma * sigma;
+#pragma unroll
+            for (int f = 0; f < 64; ++f) {
+                const float r2 = r[f] * r[f];
+                float coeff = __fdiv_rn(r2, r2 + s2);
+                if (f == 0 && c == 0) coeff = 1.0f;
+                const float val = v[f] * coeff;
+                k = fmaf_(coeff, coeff, k);
+                v[f] = val * (1.0f / 4096.0f);
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 64; ++f) {
+                const float thr = (f == 0 && c == 0) ? 0.0f : sigma;
+                const bool keep = fabsf(v[f]) >= thr;
+                k = k + (keep ? 1.0f : 0.0f);
+                v[f] = keep ? v[f] * (1.0f / 4096.0f) : 0.0f;
+            }
+        }
+        const float w = __fdiv_rn(1.0f, bfly8(k));
+        xform3d<T2, T1, false>(v, buf, c);
+
+        if (valid) {
+            const float E = a.extractor;
+            const long long pb = (long long)p * a.acc_plane_elems;
+            if (a.exact) {
+                unsigned long long *acc = (unsigned long long *)a.acc;
+                const unsigned long long qw = to_fixed((w + E) - E, a.extractor_exp);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const long long base = pb + (long long)mz[i] * 2 * TS + (long long)my[i] * S + mx[i] + c;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float u = w * v[i * 8 + j];
+                        atomicAdd(acc + base + (long long)j * S, to_fixed((u + E) - E, a.extractor_exp));
+                        atomicAdd(acc + base + (long long)j * S + TS, qw);
+                    }
+                }
+            } else {
+                float *acc = (float *)a.acc;
+                const float qw = (w + E) - E;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const long long base = pb + (long long)mz[i] * 2 * TS + (long long)my[i] * S + mx[i] + c;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float u = w * v[i * 8 + j];
+                        atomicAdd(acc + base + (long long)j * S, (u + E) - E);
+                        atomicAdd(acc + base + (long long)j * S + TS, qw);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Finishing: FP32 num/den from the accumulators; either the division (src/bm3d.cu:898-922) or
+// the stacked [P][TW][2][H][S] payload.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float acc_value(const FinishArgs &a, long long i) {
+    if (a.exact) {
+        const long long v = ((const long long *)a.acc)[i];
+        return __builtin_ldexpf((float)v, a.extractor_exp - 24);
+    }
+    return ((const float *)a.acc)[i];
+}
+
+__global__ __launch_bounds__(256) void k_finish_div(FinishArgs a) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (x >= a.width || y >= a.height || !((a.proc_mask >> p) & 1)) return;
+    const long long i = (long long)y * a.stride + x;
+    const long long pb = (long long)p * a.acc_plane_elems;
+    a.out[(long long)p * a.ts + i] = __fdiv_rn(acc_value(a, pb + i), acc_value(a, pb + a.ts + i));
+}
+
+__global__ __launch_bounds__(256) void k_finish_stack(FinishArgs a) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int pz = blockIdx.z;  // plane * (2*TW) + slot*2 + {num, den}
+    if (x >= a.width || y >= a.height) return;
+    const long long i = (long long)pz * a.ts + (long long)y * a.stride + x;
+    a.out[i] = acc_value(a, i);
+}
+
+__global__ __launch_bounds__(256) void k_zero(unsigned long long *p, long long n64) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n64; i += (long long)gridDim.x * 256) p[i] = 0ull;
+}
+
+// Temporal aggregation (src/bm3d_vagg.cu:18-48): ascending FP32 sums, one IEEE division.
+__global__ __launch_bounds__(256) void k_vaggregate(VaggArgs a) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.width || y >= a.height) return;
+    const long long i = (long long)y * a.stride + x;
+    float sv = 0.0f, sw = 0.0f;
+    for (int t = 0; t < a.tw; ++t) {
+        const float *base = a.slots[t] + (long long)a.z[t] * 2 * a.ts;
+        sv = sv + base[i];
+        sw = sw + base[a.ts + i];
+    }
+    a.dst[i] = __fdiv_rn(sv, sw);
+}
+
+}  // namespace vszk
+
+// ================================================================================================
+// Kernel-layer C ABI
+// ================================================================================================
+using namespace vszk;
+
+namespace {
+
+bool kparams_ok(const vsz_bm3d_kparams *p) {
+    if (!p) return false;
+    if (p->width < 8 || p->height < 8 || p->stride < p->width) return false;
+    if (p->block_step < 1 || p->block_step > 8 || p->bm_range < 1) return false;
+    if (p->radius < 0 || p->radius > 16 || p->ps_num < 1 || p->ps_num > 8 || p->ps_range < 1) return false;
+    if (p->metric < 0 || p->metric > 4 || p->t2d < 0 || p->t2d > 3 || p->t1d < 0 || p->t1d > 3) return false;
+    return true;
+}
+
+struct Layout {
+    int nbx, nby, P, TW;
+    long long ts, nblocks;
+    size_t match_bytes, acc_bytes;
+    bool exact;
+};
+
+Layout layout_of(const vsz_bm3d_kparams *p) {
+    Layout L{};
+    L.nbx = (p->width - 8 + p->block_step - 1) / p->block_step + 1;
+    L.nby = (p->height - 8 + p->block_step - 1) / p->block_step + 1;
+    L.P = p->chroma ? 3 : 1;
+    L.TW = 2 * p->radius + 1;
+    L.ts = (long long)p->height * p->stride;
+    L.nblocks = (long long)L.nbx * L.nby;
+    L.exact = vsz_exact_extractor(p->extractor_exp);
+    L.match_bytes = ((size_t)L.nblocks * 8 * sizeof(int4) + 255) & ~size_t(255);
+    L.acc_bytes = (size_t)L.P * L.TW * 2 * L.ts * (L.exact ? 8 : 4);
+    return L;
+}
+
+template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s) {
+    const unsigned grid = (unsigned)((a.nbx * (long long)a.nby + 255) / 256);
+    if (temporal) hipLaunchKernelGGL((k_match<M, true>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_match<M, false>), dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_match(const MatchArgs &a, int metric, bool temporal, hipStream_t s) {
+    switch (metric) {
+    case 0: launch_match_m<0>(a, temporal, s); break;
+    case 1: launch_match_m<1>(a, temporal, s); break;
+    case 2: launch_match_m<2>(a, temporal, s); break;
+    case 3: launch_match_m<3>(a, temporal, s); break;
+    default: launch_match_m<4>(a, temporal, s); break;
+    }
+}
+
+template <bool FINAL, int T2, int T1> void launch_filter_3(const FilterArgs &a, hipStream_t s) {
+    const unsigned grid = (unsigned)((a.nblocks + 7) / 8);
+    hipLaunchKernelGGL((k_filter<FINAL, T2, T1>), dim3(grid), dim3(64), 0, s, a);
+}
+
+template <bool FINAL, int T2> void launch_filter_2(const FilterArgs &a, int t1, hipStream_t s) {
+    switch (t1) {
+    case 0: launch_filter_3<FINAL, T2, 0>(a, s); break;
+    case 1: launch_filter_3<FINAL, T2, 1>(a, s); break;
+    case 2: launch_filter_3<FINAL, T2, 2>(a, s); break;
+    default: launch_filter_3<FINAL, T2, 3>(a, s); break;
+    }
+}
+
+template <bool FINAL> void launch_filter_1(const FilterArgs &a, int t2, int t1, hipStream_t s) {
+    switch (t2) {
+    case 0: launch_filter_2<FINAL, 0>(a, t1, s); break;
+    case 1: launch_filter_2<FINAL, 1>(a, t1, s); break;
+    case 2: launch_filter_2<FINAL, 2>(a, t1, s); break;
+    default: launch_filter_2<FINAL, 3>(a, t1, s); break;
+    }
+}
+
+int check_launch(const char *what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return vsz_fail(VSZ_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+    return VSZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vsz_bm3d_num_blocks(const vsz_bm3d_kparams *p, int *nbx, int *nby) {
+    if (!kparams_ok(p)) return vsz_fail(VSZ_EINVAL, "invalid kernel parameters");
+    const Layout L = layout_of(p);
+    if (nbx) *nbx = L.nbx;
+    if (nby) *nby = L.nby;
+    return VSZ_OK;
+}
+
+size_t vsz_bm3d_workspace_size(const vsz_bm3d_kparams *p) {
+    if (!kparams_ok(p)) return 0;
+    const Layout L = layout_of(p);
+    return L.match_bytes + L.acc_bytes;
+}
+
+int vsz_bm3d_match(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_matches, void *stream) {
+    if (!kparams_ok(p) || !d_img || !d_matches) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_match: invalid arguments");
+    const Layout L = layout_of(p);
+    MatchArgs m{};
+    m.img = d_img;
+    m.matches = reinterpret_cast<int4 *>(d_matches);
+    m.width = p->width;
+    m.height = p->height;
+    m.stride = p->stride;
+    m.nbx = L.nbx;
+    m.nby = L.nby;
+    m.block_step = p->block_step;
+    m.bm_range = p->bm_range;
+    m.radius = p->radius;
+    m.ps_num = p->ps_num;
+    m.ps_range = p->ps_range;
+    m.ts = L.ts;
+    launch_match(m, p->metric, p->radius > 0, (hipStream_t)stream);
+    return check_launch("vsz_bm3d_match");
+}
+
+int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, void *d_work, float *d_out,
+                 int out_kind, void *stream) {
+    if (!kparams_ok(p) || !d_clip0 || !d_work || !d_out || (p->final_pass && !d_clip1) || out_kind < 0 ||
+        out_kind > 1 || (out_kind == 0 && p->radius != 0))
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_run: invalid arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const Layout L = layout_of(p);
+    int4 *matches = reinterpret_cast<int4 *>(d_work);
+    void *acc = static_cast<char *>(d_work) + L.match_bytes;
+
+    // zero the accumulators with a kernel (graph-capture safe, no memset node)
+    {
+        const long long n64 = (long long)(L.acc_bytes / 8);
+        const unsigned grid = (unsigned)std::min<long long>((n64 + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, s, (unsigned long long *)acc, n64);
+    }
+    int rc = vsz_bm3d_match(p, d_clip0, reinterpret_cast<int32_t *>(matches), stream);
+    if (rc) return rc;
+
+    FilterArgs f{};
+    f.matches = matches;
+    f.clip0 = d_clip0;
+    f.clip1 = p->final_pass ? d_clip1 : d_clip0;
+    f.acc = acc;
+    f.nblocks = L.nblocks;
+    f.num_planes = L.P;
+    f.stride = p->stride;
+    f.ts = L.ts;
+    f.plane_elems = (long long)L.TW * L.ts;
+    f.acc_plane_elems = (long long)L.TW * 2 * L.ts;
+    unsigned active = 0;
+    for (int i = 0; i < L.P; ++i) {
+        f.sigma[i] = p->sigma[i];
+        // chroma planes whose scaled sigma is below FLT_EPSILON are skipped (src/bm3d.cu:804-810)
+        if (!(p->chroma && p->sigma[i] < FLT_EPSILON)) active |= 1u << i;
+    }
+    f.active_mask = active;
+    f.exact = L.exact ? 1 : 0;
+    f.extractor_exp = p->extractor_exp;
+    f.extractor = p->extractor_exp != 0 ? ldexpf(1.0f, p->extractor_exp) : 0.0f;
+    if (p->final_pass) launch_filter_1<true>(f, p->t2d, p->t1d, s);
+    else launch_filter_1<false>(f, p->t2d, p->t1d, s);
+    rc = check_launch("vsz_bm3d_run: filter");
+    if (rc) return rc;
+
+    FinishArgs fa{};
+    fa.acc = acc;
+    fa.out = d_out;
+    fa.width = p->width;
+    fa.height = p->height;
+    fa.stride = p->stride;
+    fa.ts = L.ts;
+    fa.acc_plane_elems = (long long)L.TW * 2 * L.ts;
+    fa.proc_mask = p->proc_mask;
+    fa.exact = L.exact ? 1 : 0;
+    fa.extractor_exp = p->extractor_exp;
+    const dim3 blk(256);
+    if (out_kind == 0) {
+        const dim3 grid((p->width + 63) / 64, (p->height + 3) / 4, L.P);
+        hipLaunchKernelGGL(k_finish_div, grid, blk, 0, s, fa);
+    } else {
+        const dim3 grid((p->width + 63) / 64, (p->height + 3) / 4, L.P * L.TW * 2);
+        hipLaunchKernelGGL(k_finish_stack, grid, blk, 0, s, fa);
+    }
+    return check_launch("vsz_bm3d_run: finish");
+}
+
+int vsz_vaggregate_run(int width, int height, int stride, int radius, int n, int num_frames,
+                       const float *const *d_slots, float *d_dst, void *stream) {
+    if (width < 1 || height < 1 || stride < width || radius < 1 || radius > 16 || n < 0 || n >= num_frames ||
+        !d_slots || !d_dst)
+        return vsz_fail(VSZ_EINVAL, "vsz_vaggregate_run: invalid arguments");
+    VaggArgs a{};
+    a.width = width;
+    a.height = height;
+    a.stride = stride;
+    a.ts = (long long)height * stride;
+    a.tw = 2 * radius + 1;
+    for (int i = 0; i < a.tw; ++i) {
+        if (!d_slots[i]) return vsz_fail(VSZ_EINVAL, "vsz_vaggregate_run: null slot");
+        a.slots[i] = d_slots[i];
+        a.z[i] = vsz_agg_z(i, n, num_frames, radius);
+    }
+    a.dst = d_dst;
+    const dim3 grid((width + 63) / 64, (height + 3) / 4, 1);
+    hipLaunchKernelGGL(k_vaggregate, grid, dim3(256), 0, (hipStream_t)stream, a);
+    return check_launch("vsz_vaggregate_run");
+}
+
+}  // extern "C"
