@@ -20,6 +20,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include "vszipcu_bm3d.h"
 #include "internal.h"
 
@@ -370,16 +372,24 @@ __global__ __launch_bounds__(256) void k_match(MatchArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Collaborative filtering + aggregation: 8 lanes per group (lane = patch column), 8 groups
-// per 64-lane workgroup. Each lane holds v[i*8 + j] = value at (rank i, row j, its column).
+// Collaborative filtering. 8 lanes per group (lane = patch column c), 8 groups per wave; each
+// lane holds v[i*8 + j] = value at (rank i, row j, its column). The horizontal pass needs rows,
+// so each rank goes through a per-wave LDS transpose buffer; a workgroup-scope fence between
+// the writes and the reads makes the wave's own LDS writes visible to its reads.
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void lds_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ void transpose8x8(float *v, float *buf, int c) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) buf[j * 9 + c] = v[j];
-    __syncthreads();
+    lds_wave_sync();
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = buf[c * 9 + j];
-    __syncthreads();
+    lds_wave_sync();
 }
 
 template <int T2, int T1, bool FWD> __device__ __forceinline__ void xform3d(float *v, float *buf, int c) {
@@ -411,20 +421,61 @@ __device__ __forceinline__ float bfly8(float x) {
     return x;
 }
 
-__device__ __forceinline__ unsigned long long to_fixed(float q, int e) {
+__device__ __forceinline__ long long to_fixed(float q, int e) {
     float s = __builtin_ldexpf(q, 24 - e);
     s = fminf(fmaxf(s, -4.611686018427387904e18f), 4.611686018427387904e18f);
-    return (unsigned long long)(long long)s;
+    return (long long)s;
 }
 
+// Loads the group (and, in the final pass, the basic-estimate group) of column c.
+template <bool FINAL>
+__device__ __forceinline__ void gather_group(const FilterArgs &a, int p, const int *mx, const int *my, const int *mz,
+                                             int c, float *v, float *r) {
+    const float *pref = a.clip0 + (long long)p * a.plane_elems;
+    const float *pnoisy = FINAL ? a.clip1 + (long long)p * a.plane_elems : pref;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const long long off = (long long)mz[i] * a.ts + (long long)my[i] * a.stride + mx[i] + c;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            v[i * 8 + j] = pnoisy[off + (long long)j * a.stride];
+            if constexpr (FINAL) r[i * 8 + j] = pref[off + (long long)j * a.stride];
+        }
+    }
+}
+
+// 3-D transform, shrinkage (src/bm3d.cu:416-546) and inverse; returns the adaptive weight.
 template <bool FINAL, int T2, int T1>
-__global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
-    __shared__ float tbuf[8][72];
-    const int lane = threadIdx.x, g = lane >> 3, c = lane & 7;
-    const int gid = blockIdx.x * 8 + g;
-    const bool valid = gid < a.nblocks;
-    const int4 *ml = a.matches + (long long)(valid ? gid : 0) * 8;
-    int mx[8], my[8], mz[8];
+__device__ __forceinline__ float filter_group(float *v, float *r, float sigma, float *buf, int c) {
+    xform3d<T2, T1, true>(v, buf, c);
+    float k = 0.0f;
+    if constexpr (FINAL) {
+        xform3d<T2, T1, true>(r, buf, c);
+        const float s2 = sigma * sigma;
+#pragma unroll
+        for (int f = 0; f < 64; ++f) {
+            const float r2 = r[f] * r[f];
+            float coeff = __fdiv_rn(r2, r2 + s2);
+            if (f == 0 && c == 0) coeff = 1.0f;
+            const float val = v[f] * coeff;
+            k = fmaf_(coeff, coeff, k);
+            v[f] = val * (1.0f / 4096.0f);
+        }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 64; ++f) {
+            const float thr = (f == 0 && c == 0) ? 0.0f : sigma;
+            const bool keep = fabsf(v[f]) >= thr;
+            k = k + (keep ? 1.0f : 0.0f);
+            v[f] = keep ? v[f] * (1.0f / 4096.0f) : 0.0f;
+        }
+    }
+    const float w = __fdiv_rn(1.0f, bfly8(k));
+    xform3d<T2, T1, false>(v, buf, c);
+    return w;
+}
+
+__device__ __forceinline__ void load_matches(const int4 *ml, int *mx, int *my, int *mz) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int4 m = ml[k];
@@ -432,115 +483,188 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
         my[k] = m.y;
         mz[k] = m.z;
     }
-    const long long TS = a.ts;
-    const int S = a.stride;
+}
+
+// Denominator scatter: a patch adds the same quantised weight to all 64 of its pixels, so the
+// kernels add it once at the patch's top-left corner; the finish kernel's exact 8x8 box sum
+// turns these corner sums into the per-pixel denominator.
+template <bool EXACT>
+__device__ __forceinline__ void scatter_den(const FilterArgs &a, long long pb, const int *mx, const int *my,
+                                            const int *mz, float qw) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const long long at = pb + (long long)mz[i] * 2 * a.ts + a.ts + (long long)my[i] * a.stride + mx[i];
+        if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)to_fixed(qw, a.extractor_exp));
+        else atomicAdd((float *)a.acc + at, qw);
+    }
+}
+
+// Direct path: every numerator contribution is a device-wide atomic (temporal passes and
+// search windows too large for an LDS tile).
+template <bool FINAL, int T2, int T1, bool EXACT>
+__global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
+    __shared__ float tbuf[8][72];
+    const int lane = threadIdx.x, g = lane >> 3, c = lane & 7;
+    const int gid = blockIdx.x * 8 + g;
+    const bool valid = gid < a.nblocks;
+    int mx[8], my[8], mz[8];
+    load_matches(a.matches + (long long)(valid ? gid : 0) * 8, mx, my, mz);
     float *buf = tbuf[g];
+    const float E = a.extractor;
 
     for (int p = 0; p < a.num_planes; ++p) {
         if (!((a.active_mask >> p) & 1)) continue;
-        const float sigma = a.sigma[p];
-        const float *pref = a.clip0 + (long long)p * a.plane_elems;
-        const float *pnoisy = FINAL ? a.clip1 + (long long)p * a.plane_elems : pref;
         float v[64];
         float r[FINAL ? 64 : 1];
+        gather_group<FINAL>(a, p, mx, my, mz, c, v, r);
+        const float w = filter_group<FINAL, T2, T1>(v, r, a.sigma[p], buf, c);
+        if (!valid) continue;
+        const long long pb = (long long)p * a.acc_plane_elems;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const long long off = (long long)mz[i] * TS + (long long)my[i] * S + mx[i] + c;
+            const long long base = pb + (long long)mz[i] * 2 * a.ts + (long long)my[i] * a.stride + mx[i] + c;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                v[i * 8 + j] = pnoisy[off + (long long)j * S];
-                if constexpr (FINAL) r[i * 8 + j] = pref[off + (long long)j * S];
+                const float q = (w * v[i * 8 + j] + E) - E;
+                if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + base + (long long)j * a.stride,
+                                               (unsigned long long)to_fixed(q, a.extractor_exp));
+                else atomicAdd((float *)a.acc + base + (long long)j * a.stride, q);
             }
         }
-        xform3d<T2, T1, true>(v, buf, c);
-        float k = 0.0f;
-        if constexpr (FINAL) {
-            xform3d<T2, T1, true>(r, buf, c);
-            const float s2 = sigma * sigma;
-#pragma unroll
-            for (int f = 0; f < 64; ++f) {
-                const float r2 = r[f] * r[f];
-                float coeff = __fdiv_rn(r2, r2 + s2);
-                if (f == 0 && c == 0) coeff = 1.0f;
-                const float val = v[f] * coeff;
-                k = fmaf_(coeff, coeff, k);
-                v[f] = val * (1.0f / 4096.0f);
-            }
-        } else {
-#pragma unroll
-            for (int f = 0; f < 64; ++f) {
-                const float thr = (f == 0 && c == 0) ? 0.0f : sigma;
-                const bool keep = fabsf(v[f]) >= thr;
-                k = k + (keep ? 1.0f : 0.0f);
-                v[f] = keep ? v[f] * (1.0f / 4096.0f) : 0.0f;
-            }
-        }
-        const float w = __fdiv_rn(1.0f, bfly8(k));
-        xform3d<T2, T1, false>(v, buf, c);
+        if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
+    }
+}
 
-        if (valid) {
-            const float E = a.extractor;
-            const long long pb = (long long)p * a.acc_plane_elems;
-            if (a.exact) {
-                unsigned long long *acc = (unsigned long long *)a.acc;
-                const unsigned long long qw = to_fixed((w + E) - E, a.extractor_exp);
+// Tiled path (radius 0): a 256-thread workgroup owns a tbx x tby tile of reference blocks and
+// accumulates the numerators of all their groups in an LDS image of the region the groups can
+// reach (the tile's search windows). With a non-zero extractor the LDS cells are exact 32-bit
+// fixed-point partial sums; a contribution whose magnitude could overflow them (at most
+// 8*tbx*tby contributions per cell, each below 2^guard_bits) or that falls outside the region
+// (fill entries of tiny windows) goes straight to the device accumulator. Each plane's region
+// is flushed once with 64-bit device atomics; integer addition commutes, so the result does
+// not depend on the order of any of these updates.
+template <bool FINAL, int T2, int T1, bool EXACT>
+__global__ __launch_bounds__(256) void k_filter_tile(FilterArgs a) {
+    extern __shared__ float smem[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 3, c = lane & 7;
+    float *buf = smem + (wave * 8 + g) * 72;
+    int *cell_i = reinterpret_cast<int *>(smem + 4 * 8 * 72);
+    float *cell_f = smem + 4 * 8 * 72;
+
+    const int W = a.width, H = a.height, bs = a.block_step, R = a.bm_range;
+    const int bx0 = blockIdx.x * a.tbx, by0 = blockIdx.y * a.tby;
+    const int nbx_t = min(a.tbx, a.nbx - bx0), nby_t = min(a.tby, a.nby - by0);
+    const int ngroups = nbx_t * nby_t;
+    const int rx0 = max(min(bx0 * bs, W - 8) - R, 0);
+    const int rx1 = min(min((bx0 + nbx_t - 1) * bs, W - 8) + R, W - 8) + 8;
+    const int ry0 = max(min(by0 * bs, H - 8) - R, 0);
+    const int ry1 = min(min((by0 + nby_t - 1) * bs, H - 8) + R, H - 8) + 8;
+    const int rw = rx1 - rx0, cells = rw * (ry1 - ry0);
+    const long long guard = 1ll << a.guard_bits;
+    const float E = a.extractor;
+
+    for (int i = threadIdx.x; i < cells; i += 256) cell_i[i] = 0;
+    __syncthreads();
+
+    for (int p = 0; p < a.num_planes; ++p) {
+        if (!((a.active_mask >> p) & 1)) continue;
+        const long long pb = (long long)p * a.acc_plane_elems;
+        for (int base = wave * 8; base < ngroups; base += 32) {
+            const int gi = base + g;
+            const bool valid = gi < ngroups;
+            const int gs = valid ? gi : 0;
+            const long long gid = (long long)(by0 + gs / nbx_t) * a.nbx + bx0 + gs % nbx_t;
+            int mx[8], my[8], mz[8];
+            load_matches(a.matches + gid * 8, mx, my, mz);
+            float v[64];
+            float r[FINAL ? 64 : 1];
+            gather_group<FINAL>(a, p, mx, my, mz, c, v, r);
+            const float w = filter_group<FINAL, T2, T1>(v, r, a.sigma[p], buf, c);
+            if (!valid) continue;
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const long long base = pb + (long long)mz[i] * 2 * TS + (long long)my[i] * S + mx[i] + c;
+            for (int i = 0; i < 8; ++i) {
+                const bool inside = mx[i] >= rx0 && mx[i] + 8 <= rx1 && my[i] >= ry0 && my[i] + 8 <= ry1;
+                const int cell0 = (my[i] - ry0) * rw + mx[i] - rx0 + c;
+                const long long gbase = pb + (long long)my[i] * a.stride + mx[i] + c;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const float u = w * v[i * 8 + j];
-                        atomicAdd(acc + base + (long long)j * S, to_fixed((u + E) - E, a.extractor_exp));
-                        atomicAdd(acc + base + (long long)j * S + TS, qw);
+                for (int j = 0; j < 8; ++j) {
+                    const float q = (w * v[i * 8 + j] + E) - E;
+                    if constexpr (EXACT) {
+                        const long long f = to_fixed(q, a.extractor_exp);
+                        if (inside && f < guard && f > -guard) atomicAdd(cell_i + cell0 + j * rw, (int)f);
+                        else atomicAdd((unsigned long long *)a.acc + gbase + (long long)j * a.stride, (unsigned long long)f);
+                    } else {
+                        if (inside) atomicAdd(cell_f + cell0 + j * rw, q);
+                        else atomicAdd((float *)a.acc + gbase + (long long)j * a.stride, q);
                     }
                 }
+            }
+            if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < cells; i += 256) {
+            const long long at = pb + (long long)(ry0 + i / rw) * a.stride + rx0 + i % rw;
+            if constexpr (EXACT) {
+                const int vv = cell_i[i];
+                if (vv) atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)(long long)vv);
+                cell_i[i] = 0;
             } else {
-                float *acc = (float *)a.acc;
-                const float qw = (w + E) - E;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const long long base = pb + (long long)mz[i] * 2 * TS + (long long)my[i] * S + mx[i] + c;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const float u = w * v[i * 8 + j];
-                        atomicAdd(acc + base + (long long)j * S, (u + E) - E);
-                        atomicAdd(acc + base + (long long)j * S + TS, qw);
-                    }
-                }
+                const float vv = cell_f[i];
+                if (vv != 0.0f) atomicAdd((float *)a.acc + at, vv);
+                cell_f[i] = 0.0f;
             }
         }
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Finishing: FP32 num/den from the accumulators; either the division (src/bm3d.cu:898-922) or
-// the stacked [P][TW][2][H][S] payload.
+// Finishing: numerator as accumulated, denominator as the exact 8x8 box sum of the corner
+// scatter, each rounded once to FP32; then either the IEEE division (src/bm3d.cu:898-922) or
+// the stacked [P][TW][2][H][S] payload. One workgroup per 64x16 output tile.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ float acc_value(const FinishArgs &a, long long i) {
-    if (a.exact) {
-        const long long v = ((const long long *)a.acc)[i];
-        return __builtin_ldexpf((float)v, a.extractor_exp - 24);
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_finish(FinishArgs a) {
+    using T = typename std::conditional<EXACT, long long, float>::type;
+    __shared__ T sD[23][72];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 16, pz = blockIdx.z;
+    const int p = pz / a.tw;
+    if (a.out_kind == 0 && !((a.proc_mask >> p) & 1)) return;
+    const long long base = (long long)pz * 2 * a.ts;
+    const T *num = reinterpret_cast<const T *>(a.acc) + base;
+    const T *D = num + a.ts;
+    for (int idx = threadIdx.x; idx < 23 * 71; idx += 256) {
+        const int rr = idx / 71, cc = idx % 71;
+        const int y = y0 - 7 + rr, x = x0 - 7 + cc;
+        sD[rr][cc] = (y >= 0 && y < a.height && x >= 0 && x < a.width) ? D[(long long)y * a.stride + x] : T(0);
     }
-    return ((const float *)a.acc)[i];
-}
-
-__global__ __launch_bounds__(256) void k_finish_div(FinishArgs a) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int p = blockIdx.z;
-    if (x >= a.width || y >= a.height || !((a.proc_mask >> p) & 1)) return;
-    const long long i = (long long)y * a.stride + x;
-    const long long pb = (long long)p * a.acc_plane_elems;
-    a.out[(long long)p * a.ts + i] = __fdiv_rn(acc_value(a, pb + i), acc_value(a, pb + a.ts + i));
-}
-
-__global__ __launch_bounds__(256) void k_finish_stack(FinishArgs a) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int pz = blockIdx.z;  // plane * (2*TW) + slot*2 + {num, den}
-    if (x >= a.width || y >= a.height) return;
-    const long long i = (long long)pz * a.ts + (long long)y * a.stride + x;
-    a.out[i] = acc_value(a, i);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int px = threadIdx.x & 63, py = (threadIdx.x >> 6) + 4 * k;
+        const int x = x0 + px, y = y0 + py;
+        if (x >= a.width || y >= a.height) continue;
+        T s = T(0);
+#pragma unroll
+        for (int dy = 0; dy < 8; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 8; ++dx) s += sD[py + dy][px + dx];
+        const long long i = (long long)y * a.stride + x;
+        float fn, fd;
+        if constexpr (EXACT) {
+            fn = __builtin_ldexpf((float)num[i], a.extractor_exp - 24);
+            fd = __builtin_ldexpf((float)s, a.extractor_exp - 24);
+        } else {
+            fn = num[i];
+            fd = s;
+        }
+        if (a.out_kind == 0) {
+            a.out[(long long)p * a.ts + i] = __fdiv_rn(fn, fd);
+        } else {
+            a.out[base + i] = fn;
+            a.out[base + a.ts + i] = fd;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_zero(unsigned long long *p, long long n64) {
@@ -617,26 +741,84 @@ void launch_match(const MatchArgs &a, int metric, bool temporal, hipStream_t s) 
     }
 }
 
-template <bool FINAL, int T2, int T1> void launch_filter_3(const FilterArgs &a, hipStream_t s) {
-    const unsigned grid = (unsigned)((a.nblocks + 7) / 8);
-    hipLaunchKernelGGL((k_filter<FINAL, T2, T1>), dim3(grid), dim3(64), 0, s, a);
+// LDS budget of the tiled filter kernel: per-wave transpose buffers + the region image.
+constexpr int kTileThreads = 256;
+constexpr size_t kTransposeBytes = 4 * 8 * 72 * sizeof(float);
+constexpr size_t kRegionBudgetBytes = 64 * 1024;
+
+struct TilePlan {
+    bool ok = false;
+    int tbx = 0, tby = 0, guard_bits = 0;
+    size_t lds_bytes = 0;
+};
+
+// Picks the block tile of the LDS-accumulating filter kernel: the region a tile's groups can
+// reach must fit the LDS budget; among those, the tiling with the least flush traffic (tiles x
+// region cells) that still yields enough workgroups to fill the chip.
+TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L) {
+    TilePlan best;
+    if (p->radius != 0) return best;
+    const int opts[] = {32, 16, 8, 4, 2, 1};
+    long long best_cost = -1;
+    for (int tbx : opts)
+        for (int tby : opts) {
+            if (tbx > L.nbx * 2 && tbx > 1) continue;
+            if (tby > L.nby * 2 && tby > 1) continue;
+            if (tbx * tby > 512) continue;
+            const long long rw = std::min<long long>((long long)(tbx - 1) * p->block_step + 8 + 2LL * p->bm_range, p->width);
+            const long long rh = std::min<long long>((long long)(tby - 1) * p->block_step + 8 + 2LL * p->bm_range, p->height);
+            const size_t bytes = kTransposeBytes + (size_t)(rw * rh) * 4;
+            if (bytes > kRegionBudgetBytes) continue;
+            const long long wgs = (long long)((L.nbx + tbx - 1) / tbx) * ((L.nby + tby - 1) / tby);
+            long long cost = wgs * rw * rh;
+            if (wgs < 1024) cost += (1024 - wgs) * rw * rh * 4;  // under-filled chip
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                best.ok = true;
+                best.tbx = tbx;
+                best.tby = tby;
+                best.lds_bytes = bytes;
+            }
+        }
+    if (best.ok) {
+        int lg = 0;
+        while ((1LL << lg) < 8LL * best.tbx * best.tby) ++lg;
+        best.guard_bits = 31 - lg;  // (8*tbx*tby) * 2^guard_bits <= 2^31
+    }
+    return best;
 }
 
-template <bool FINAL, int T2> void launch_filter_2(const FilterArgs &a, int t1, hipStream_t s) {
-    switch (t1) {
-    case 0: launch_filter_3<FINAL, T2, 0>(a, s); break;
-    case 1: launch_filter_3<FINAL, T2, 1>(a, s); break;
-    case 2: launch_filter_3<FINAL, T2, 2>(a, s); break;
-    default: launch_filter_3<FINAL, T2, 3>(a, s); break;
+template <bool FINAL, int T2, int T1, bool EXACT>
+void launch_filter_4(const FilterArgs &a, const TilePlan &tp, hipStream_t s) {
+    if (tp.ok) {
+        const dim3 grid((a.nbx + tp.tbx - 1) / tp.tbx, (a.nby + tp.tby - 1) / tp.tby, 1);
+        hipLaunchKernelGGL((k_filter_tile<FINAL, T2, T1, EXACT>), grid, dim3(kTileThreads), tp.lds_bytes, s, a);
+    } else {
+        const unsigned grid = (unsigned)((a.nblocks + 7) / 8);
+        hipLaunchKernelGGL((k_filter<FINAL, T2, T1, EXACT>), dim3(grid), dim3(64), 0, s, a);
     }
 }
 
-template <bool FINAL> void launch_filter_1(const FilterArgs &a, int t2, int t1, hipStream_t s) {
+template <bool FINAL, int T2, int T1> void launch_filter_3(const FilterArgs &a, const TilePlan &tp, hipStream_t s) {
+    if (a.exact) launch_filter_4<FINAL, T2, T1, true>(a, tp, s);
+    else launch_filter_4<FINAL, T2, T1, false>(a, tp, s);
+}
+
+template <bool FINAL, int T2> void launch_filter_2(const FilterArgs &a, const TilePlan &tp, int t1, hipStream_t s) {
+    switch (t1) {
+    case 0: launch_filter_3<FINAL, T2, 0>(a, tp, s); break;
+    case 1: launch_filter_3<FINAL, T2, 1>(a, tp, s); break;
+    case 2: launch_filter_3<FINAL, T2, 2>(a, tp, s); break;
+    default: launch_filter_3<FINAL, T2, 3>(a, tp, s); break;
+    }
+}
+
+template <bool FINAL> void launch_filter_1(const FilterArgs &a, const TilePlan &tp, int t2, int t1, hipStream_t s) {
     switch (t2) {
-    case 0: launch_filter_2<FINAL, 0>(a, t1, s); break;
-    case 1: launch_filter_2<FINAL, 1>(a, t1, s); break;
-    case 2: launch_filter_2<FINAL, 2>(a, t1, s); break;
-    default: launch_filter_2<FINAL, 3>(a, t1, s); break;
+    case 0: launch_filter_2<FINAL, 0>(a, tp, t1, s); break;
+    case 1: launch_filter_2<FINAL, 1>(a, tp, t1, s); break;
+    case 2: launch_filter_2<FINAL, 2>(a, tp, t1, s); break;
+    default: launch_filter_2<FINAL, 3>(a, tp, t1, s); break;
     }
 }
 
@@ -725,8 +907,18 @@ int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d
     f.exact = L.exact ? 1 : 0;
     f.extractor_exp = p->extractor_exp;
     f.extractor = p->extractor_exp != 0 ? ldexpf(1.0f, p->extractor_exp) : 0.0f;
-    if (p->final_pass) launch_filter_1<true>(f, p->t2d, p->t1d, s);
-    else launch_filter_1<false>(f, p->t2d, p->t1d, s);
+    f.width = p->width;
+    f.height = p->height;
+    f.block_step = p->block_step;
+    f.bm_range = p->bm_range;
+    f.nbx = L.nbx;
+    f.nby = L.nby;
+    const TilePlan tp = plan_tiles(p, L);
+    f.tbx = tp.tbx;
+    f.tby = tp.tby;
+    f.guard_bits = tp.guard_bits;
+    if (p->final_pass) launch_filter_1<true>(f, tp, p->t2d, p->t1d, s);
+    else launch_filter_1<false>(f, tp, p->t2d, p->t1d, s);
     rc = check_launch("vsz_bm3d_run: filter");
     if (rc) return rc;
 
@@ -736,19 +928,14 @@ int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d
     fa.width = p->width;
     fa.height = p->height;
     fa.stride = p->stride;
+    fa.tw = L.TW;
+    fa.out_kind = out_kind;
     fa.ts = L.ts;
-    fa.acc_plane_elems = (long long)L.TW * 2 * L.ts;
     fa.proc_mask = p->proc_mask;
-    fa.exact = L.exact ? 1 : 0;
     fa.extractor_exp = p->extractor_exp;
-    const dim3 blk(256);
-    if (out_kind == 0) {
-        const dim3 grid((p->width + 63) / 64, (p->height + 3) / 4, L.P);
-        hipLaunchKernelGGL(k_finish_div, grid, blk, 0, s, fa);
-    } else {
-        const dim3 grid((p->width + 63) / 64, (p->height + 3) / 4, L.P * L.TW * 2);
-        hipLaunchKernelGGL(k_finish_stack, grid, blk, 0, s, fa);
-    }
+    const dim3 grid((p->width + 63) / 64, (p->height + 15) / 16, L.P * L.TW);
+    if (L.exact) hipLaunchKernelGGL(k_finish<true>, grid, dim3(256), 0, s, fa);
+    else hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, fa);
     return check_launch("vsz_bm3d_run: finish");
 }
 

@@ -35,7 +35,7 @@ struct FilterArgs {
     const int4 *matches;
     const float *clip0;  // [P][TW][H][S] basic (final) or noisy (basic pass)
     const float *clip1;  // [P][TW][H][S] noisy (final pass)
-    void *acc;           // [P][TW][2][H][S] int64 (exact) or float
+    void *acc;           // [P][TW][2][H][S] int64 (exact) or float: numerator, corner-scattered weights
     long long nblocks;
     int num_planes, stride;
     long long ts, plane_elems, acc_plane_elems;
@@ -43,15 +43,18 @@ struct FilterArgs {
     unsigned active_mask;
     int exact, extractor_exp;
     float extractor;
+    // tiled path
+    int width, height, block_step, bm_range, nbx, nby;
+    int tbx, tby, guard_bits;
 };
 
 struct FinishArgs {
     const void *acc;
     float *out;
-    int width, height, stride;
-    long long ts, acc_plane_elems;
+    int width, height, stride, tw, out_kind;
+    long long ts;
     unsigned proc_mask;
-    int exact, extractor_exp;
+    int extractor_exp;
 };
 
 // Passed by value, like the reference's AggSrc (src/bm3d_vagg.cu:7-16).
