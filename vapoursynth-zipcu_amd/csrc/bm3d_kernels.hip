@@ -157,20 +157,70 @@ __device__ __forceinline__ float tree8(const float *c) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Block distances (src/bm3d.cu:290-390). `a` is the reference patch (row-major 8x8 in
-// registers), `b` the candidate's top-left in the image.
+// Block matching (src/bm3d.cu:579-788). One lane per reference block; a 64-lane workgroup
+// covers an 8x8 arrangement of blocks. A search row is scanned left to right with the 8x8
+// candidate held in registers: each step loads one new column (8 values) into the slot the
+// leftmost column vacates; the slots rotate with the step index mod 8, which a static
+// unroll-by-8 turns into fixed register names. Every lane walks the same displacement grid;
+// candidates outside its clipped window are skipped (their loads use clamped addresses).
 // ------------------------------------------------------------------------------------------------
-template <int M>
-__device__ __forceinline__ float block_dist(const float *a, const float *__restrict__ b, int S) {
-    float col[8];
-    if constexpr (M == 0 || M == 1) {
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): workgroup g
+// runs on XCD g % 8. Remap so that XCD k processes tiles [k*n/8, (k+1)*n/8), a contiguous band
+// whose search windows overlap in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int g, int n) {
+    const int xcd = g & 7, slot = g >> 3;
+    const int per = n >> 3, extra = n & 7;  // XCDs below `extra` take one more tile
+    return xcd * per + min(xcd, extra) + slot;
+}
+
+// Per-lane reference patch a[r*8 + c] and the metric's reference-only terms.
+struct RefPatch {
+    float a[64];
+    float mean;  // zssd / zsad: tree-summed patch mean (src/bm3d.cu:319-321)
+    float norm;  // ssd/norm: sqrt of the tree-summed squares (src/bm3d.cu:363-369)
+};
+
+__device__ __forceinline__ float col_sum8(const float *p, int stride) {
+    return ((p[0] + p[stride]) + (p[2 * stride] + p[3 * stride])) +
+           ((p[4 * stride] + p[5 * stride]) + (p[6 * stride] + p[7 * stride]));
+}
+
+template <int M> __device__ __forceinline__ void ref_terms(RefPatch &R) {
+    if constexpr (M == 2 || M == 3) {
+        float ca[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) ca[c] = col_sum8(R.a + c, 8);
+        R.mean = tree8(ca) * (1.0f / 64.0f);
+    } else if constexpr (M == 4) {
+        float qa[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             float e0 = 0.0f, e1 = 0.0f;
 #pragma unroll
             for (int r = 0; r < 8; r += 2) {
-                const float d0 = a[r * 8 + c] - b[r * S + c];
-                const float d1 = a[(r + 1) * 8 + c] - b[(r + 1) * S + c];
+                e0 = fmaf_(R.a[r * 8 + c], R.a[r * 8 + c], e0);
+                e1 = fmaf_(R.a[(r + 1) * 8 + c], R.a[(r + 1) * 8 + c], e1);
+            }
+            qa[c] = e0 + e1;
+        }
+        R.norm = __builtin_sqrtf(tree8(qa));  // IEEE sqrt (__fsqrt_rn is the native approximation here)
+    }
+}
+
+// Distance (src/bm3d.cu:290-390) of the reference patch to the candidate whose column c sits in
+// window buffer (U + c) % 9; w[buffer][row].
+template <int M, int U> __device__ __forceinline__ float win_dist(const RefPatch &R, const float (&w)[9][8]) {
+    float col[8];
+    if constexpr (M == 0 || M == 1) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float *b = w[(U + c) % 9];
+            float e0 = 0.0f, e1 = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 8; r += 2) {
+                const float d0 = R.a[r * 8 + c] - b[r];
+                const float d1 = R.a[(r + 1) * 8 + c] - b[r + 1];
                 if constexpr (M == 0) {
                     e0 = fmaf_(d0, d0, e0);
                     e1 = fmaf_(d1, d1, e1);
@@ -183,24 +233,18 @@ __device__ __forceinline__ float block_dist(const float *a, const float *__restr
         }
         return tree8(col);
     } else if constexpr (M == 2 || M == 3) {
-        float ca[8], cb[8], bb[64];
+        float cb[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) bb[r * 8 + c] = b[r * S + c];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            ca[c] = ((a[c] + a[8 + c]) + (a[16 + c] + a[24 + c])) + ((a[32 + c] + a[40 + c]) + (a[48 + c] + a[56 + c]));
-            cb[c] = ((bb[c] + bb[8 + c]) + (bb[16 + c] + bb[24 + c])) + ((bb[32 + c] + bb[40 + c]) + (bb[48 + c] + bb[56 + c]));
-        }
-        const float dm = tree8(ca) * (1.0f / 64.0f) - tree8(cb) * (1.0f / 64.0f);
+        for (int c = 0; c < 8; ++c) cb[c] = col_sum8(w[(U + c) % 9], 1);
+        const float dm = R.mean - tree8(cb) * (1.0f / 64.0f);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
+            const float *b = w[(U + c) % 9];
             float e0 = 0.0f, e1 = 0.0f;
 #pragma unroll
             for (int r = 0; r < 8; r += 2) {
-                const float d0 = (a[r * 8 + c] - bb[r * 8 + c]) - dm;
-                const float d1 = (a[(r + 1) * 8 + c] - bb[(r + 1) * 8 + c]) - dm;
+                const float d0 = (R.a[r * 8 + c] - b[r]) - dm;
+                const float d1 = (R.a[(r + 1) * 8 + c] - b[r + 1]) - dm;
                 if constexpr (M == 2) {
                     e0 = fmaf_(d0, d0, e0);
                     e1 = fmaf_(d1, d1, e1);
@@ -213,162 +257,343 @@ __device__ __forceinline__ float block_dist(const float *a, const float *__restr
         }
         return tree8(col);
     } else {  // ssd/norm
-        float qa[8], qb[8];
+        float qb[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            float a0 = 0.0f, a1 = 0.0f, b0 = 0.0f, b1 = 0.0f, x0 = 0.0f, x1 = 0.0f;
+            const float *b = w[(U + c) % 9];
+            float b0 = 0.0f, b1 = 0.0f, x0 = 0.0f, x1 = 0.0f;
 #pragma unroll
             for (int r = 0; r < 8; r += 2) {
-                const float va0 = a[r * 8 + c], vb0 = b[r * S + c];
-                const float va1 = a[(r + 1) * 8 + c], vb1 = b[(r + 1) * S + c];
-                a0 = fmaf_(va0, va0, a0);
-                b0 = fmaf_(vb0, vb0, b0);
-                x0 = fmaf_(va0, vb0, x0);
-                a1 = fmaf_(va1, va1, a1);
-                b1 = fmaf_(vb1, vb1, b1);
-                x1 = fmaf_(va1, vb1, x1);
+                b0 = fmaf_(b[r], b[r], b0);
+                x0 = fmaf_(R.a[r * 8 + c], b[r], x0);
+                b1 = fmaf_(b[r + 1], b[r + 1], b1);
+                x1 = fmaf_(R.a[(r + 1) * 8 + c], b[r + 1], x1);
             }
-            qa[c] = a0 + a1;
             qb[c] = b0 + b1;
             col[c] = x0 + x1;
         }
-        const float na = __builtin_sqrtf(tree8(qa));  // IEEE sqrt (__fsqrt_rn is the native approximation here)
         const float nb = __builtin_sqrtf(tree8(qb));
-        return 2.0f - __fdiv_rn(2.0f * tree8(col), (na * nb) + FLT_EPSILON);
+        return 2.0f - __fdiv_rn(2.0f * tree8(col), (R.norm * nb) + FLT_EPSILON);
     }
 }
 
-// Strict-< sorted insertion into an ascending 8-list (src/bm3d.cu:641-653).
-__device__ __forceinline__ void insert8(float (&er)[8], int (&xs)[8], int (&ys)[8], int (&zs)[8], float e, int x,
-                                        int y, int z) {
-    if (e < er[7]) {
+// Sorted 8-list of (distance, position, slot): position packs x | y << 16 (planes are below
+// 65536 in both dimensions); the slot array is carried only by temporal passes.
+struct List8 {
+    float er[8];
+    unsigned pos[8];
+    int zs[8];
+};
+
+__device__ __forceinline__ unsigned pack_xy(int x, int y) { return (unsigned)x | ((unsigned)y << 16); }
+__device__ __forceinline__ int pos_x(unsigned p) { return (int)(p & 0xffffu); }
+__device__ __forceinline__ int pos_y(unsigned p) { return (int)(p >> 16); }
+
+template <bool TZ> __device__ __forceinline__ void list_init(List8 &L, int z) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        L.er[k] = FLT_MAX;
+        L.pos[k] = 0u;
+        if constexpr (TZ) L.zs[k] = z;
+    }
+}
+
+// Strict-< sorted insertion (src/bm3d.cu:641-653): a new entry goes before the first entry
+// with a strictly greater distance, so ties keep the earlier scan position in front.
+template <bool TZ>
+__device__ __forceinline__ void insert8(List8 &L, float e, unsigned pos, int z) {
+    if (e < L.er[7]) {
 #pragma unroll
         for (int k = 7; k > 0; --k) {
-            const bool fk = e < er[k];
-            const bool fp = e < er[k - 1];
+            const bool fk = e < L.er[k];
+            const bool fp = e < L.er[k - 1];
             if (fk) {
-                er[k] = fp ? er[k - 1] : e;
-                xs[k] = fp ? xs[k - 1] : x;
-                ys[k] = fp ? ys[k - 1] : y;
-                zs[k] = fp ? zs[k - 1] : z;
+                L.er[k] = fp ? L.er[k - 1] : e;
+                L.pos[k] = fp ? L.pos[k - 1] : pos;
+                if constexpr (TZ) L.zs[k] = fp ? L.zs[k - 1] : z;
             }
         }
-        if (e < er[0]) {
-            er[0] = e;
-            xs[0] = x;
-            ys[0] = y;
-            zs[0] = z;
+        if (e < L.er[0]) {
+            L.er[0] = e;
+            L.pos[0] = pos;
+            if constexpr (TZ) L.zs[0] = z;
         }
     }
 }
 
-// ------------------------------------------------------------------------------------------------
-// Matching: one lane per reference block (v1: direct loads, no LDS staging).
-// ------------------------------------------------------------------------------------------------
-template <int M, bool TEMPORAL>
-__global__ __launch_bounds__(256) void k_match(MatchArgs a) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= a.nbx * a.nby) return;
-    const int W = a.width, H = a.height, S = a.stride, R = TEMPORAL ? a.radius : 0;
-    const int x = min((gid % a.nbx) * a.block_step, W - 8);
-    const int y = min((gid / a.nbx) * a.block_step, H - 8);
-    const float *centre = a.img + (long long)R * a.ts;
+// Raw buffer view of one [TW][H][S] plane stack: loads take a 32-bit byte offset per lane and an
+// immediate; offsets outside [0, bytes) return 0 instead of faulting, so candidates beyond the
+// plane edge need no address clamping (their distances are discarded).
+struct PlaneBuf {
+    __amdgpu_buffer_rsrc_t rsrc;
+};
 
-    float ref[64];
+__device__ __forceinline__ PlaneBuf make_plane_buf(const float *base, unsigned bytes) {
+    // dword3 0x00020000: raw (untyped) 32-bit buffer access on gfx9-family targets
+    return PlaneBuf{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)bytes, 0x00020000)};
+}
+
+template <int IMM> __device__ __forceinline__ float buf_load(const PlaneBuf &b, int voff) {
+    return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(b.rsrc, voff + IMM, 0, 0));
+}
+
+// Loads column (step U + J) of the 8 rows whose byte offsets (at the block's first column) are vo[r].
+template <int IMM> __device__ __forceinline__ void load_col(float (&col)[8], const PlaneBuf &b, const int (&vo)[8]) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) col[r] = buf_load<IMM>(b, vo[r]);
+}
+
+// One step of a search row: issue the load of the column the NEXT step needs (it has the whole
+// distance computation to arrive), then evaluate this step's candidate.
+template <int M, bool TZ, int U>
+__device__ __forceinline__ void scan_step(float (&w)[9][8], const PlaneBuf &b, const int (&vo)[8], int W, int xt,
+                                          bool row_ok, unsigned ybits, int z, const RefPatch &R, List8 &L) {
+    load_col<(U + 8) * 4>(w[(U + 8) % 9], b, vo);
+    const float e = win_dist<M, U>(R, w);
+    if (row_ok && xt >= 0 && xt <= W - 8) insert8<TZ>(L, e, (unsigned)xt | ybits, z);
+}
+
+// Scans the window [cx-rng, cx+rng] x [cy-rng, cy+rng] of frame z, clipped to the plane, in the
+// reference's row-major order (src/bm3d.cu:618-660), inserting into L with slot index z. Column
+// j of the row lives in buffer j % 9: eight are the current candidate, one is in flight.
+template <int M, bool TZ>
+__device__ __forceinline__ void scan_window(const PlaneBuf &b, long long frame_off, int S, int W, int H,
+                                            const RefPatch &R, int cx, int cy, int rng, int z, List8 &L) {
+    const int span = 2 * rng;
+    const int x0 = cx - rng;
+    for (int dy = -rng; dy <= rng; ++dy) {
+        const int yy = cy + dy;
+        const bool row_ok = yy >= 0 && yy <= H - 8;
+        const unsigned ybits = (unsigned)yy << 16;
+        int vo[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) vo[r] = (int)((frame_off + (long long)(yy + r) * S + x0) * 4);
+        // first 8 columns: separately clamped offsets, so no merged vector access can straddle
+        // the buffer start (a partially out-of-range vector load returns all zeros)
+        float w[9][8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int dj = 4 * (max(x0 + j, 0) - x0);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) w[j][r] = buf_load<0>(b, vo[r] + dj);
+        }
+        for (int tb = 0; tb <= span; tb += 9) {
+            const int xt = x0 + tb;
+            scan_step<M, TZ, 0>(w, b, vo, W, xt + 0, row_ok, ybits, z, R, L);
+            if (tb + 1 > span) break;
+            scan_step<M, TZ, 1>(w, b, vo, W, xt + 1, row_ok, ybits, z, R, L);
+            if (tb + 2 > span) break;
+            scan_step<M, TZ, 2>(w, b, vo, W, xt + 2, row_ok, ybits, z, R, L);
+            if (tb + 3 > span) break;
+            scan_step<M, TZ, 3>(w, b, vo, W, xt + 3, row_ok, ybits, z, R, L);
+            if (tb + 4 > span) break;
+            scan_step<M, TZ, 4>(w, b, vo, W, xt + 4, row_ok, ybits, z, R, L);
+            if (tb + 5 > span) break;
+            scan_step<M, TZ, 5>(w, b, vo, W, xt + 5, row_ok, ybits, z, R, L);
+            if (tb + 6 > span) break;
+            scan_step<M, TZ, 6>(w, b, vo, W, xt + 6, row_ok, ybits, z, R, L);
+            if (tb + 7 > span) break;
+            scan_step<M, TZ, 7>(w, b, vo, W, xt + 7, row_ok, ybits, z, R, L);
+            if (tb + 8 > span) break;
+            scan_step<M, TZ, 8>(w, b, vo, W, xt + 8, row_ok, ybits, z, R, L);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) vo[r] += 36;
+        }
+    }
+}
+
+// ---- LDS-staged spatial matching -------------------------------------------------------------
+// A 256-lane workgroup owns 16x16 reference blocks (2x2 waves of 8x8). It stages the union of their
+// search windows, [x_first-R, x_last+R+9) x [y_first-R, y_last+R+8) of the plane, in LDS once
+// (pixels outside the plane are stored as 0; only invalid candidates read them), then every lane
+// scans with LDS reads at immediate offsets. Row Y of the region starts at Y*LS + ((Y>>2)&3):
+// lanes of one wave are 4 rows apart for block_step 4, so the row-dependent shift spreads a
+// wave's column reads over all banks.
+__device__ __forceinline__ int lds_row(int Y, int LS) { return Y * LS + ((Y >> 2) & 3); }
+
+template <int IMM> __device__ __forceinline__ void lds_col(float (&col)[8], const char *lds, const int (&vo)[8]) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) col[r] = *reinterpret_cast<const float *>(lds + vo[r] + IMM);
+}
+
+template <int M, int U>
+__device__ __forceinline__ void lds_step(float (&w)[9][8], const char *lds, const int (&vo)[8], int W, int xt,
+                                         bool row_ok, unsigned ybits, const RefPatch &R, List8 &L) {
+    lds_col<(U + 8) * 4>(w[(U + 8) % 9], lds, vo);
+    const float e = win_dist<M, U>(R, w);
+    if (row_ok && xt >= 0 && xt <= W - 8) insert8<false>(L, e, (unsigned)xt | ybits, 0);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
+    extern __shared__ float region[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int W = a.width, H = a.height, S = a.stride, bs = a.block_step, Rg = a.bm_range;
+    const int bx0 = tx * 16, by0 = ty * 16;
+    const int x_first = min(bx0 * bs, W - 8), x_last = min(min(bx0 + 15, a.nbx - 1) * bs, W - 8);
+    const int y_first = min(by0 * bs, H - 8), y_last = min(min(by0 + 15, a.nby - 1) * bs, H - 8);
+    const int rx0 = x_first - Rg, ry0 = y_first - Rg;
+    const int RW = x_last - x_first + 2 * Rg + 9, RH = y_last - y_first + 2 * Rg + 8;
+    const int LS = a.lds_stride;
+
+    for (int i = threadIdx.x; i < RW * RH; i += 256) {
+        const int Y = i / RW, X = i % RW;
+        const int gy = ry0 + Y, gx = rx0 + X;
+        region[lds_row(Y, LS) + X] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? a.img[(long long)gy * S + gx] : 0.0f;
+    }
+    __syncthreads();
+
+    const int bx = bx0 + (wave & 1) * 8 + (lane & 7), by = by0 + (wave >> 1) * 8 + (lane >> 3);
+    const bool valid = bx < a.nbx && by < a.nby;
+    const int x = min(min(bx, a.nbx - 1) * bs, W - 8);
+    const int y = min(min(by, a.nby - 1) * bs, H - 8);
+    RefPatch R;
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) ref[r * 8 + c] = centre[(long long)(y + r) * S + x + c];
+        for (int c = 0; c < 8; ++c) R.a[r * 8 + c] = a.img[(long long)(y + r) * S + x + c];
+    ref_terms<M>(R);
+    List8 L;
+    list_init<false>(L, 0);
 
-    float er[8];
-    int xs[8], ys[8], zs[8];
+    const char *lds = reinterpret_cast<const char *>(region);
+    const int span = 2 * Rg;
+    const int X0 = x - Rg - rx0;
+    for (int dy = -Rg; dy <= Rg; ++dy) {
+        const int yy = y + dy;
+        const bool row_ok = yy >= 0 && yy <= H - 8;
+        const unsigned ybits = (unsigned)yy << 16;
+        int vo[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        er[k] = FLT_MAX;
-        xs[k] = 0;
-        ys[k] = 0;
-        zs[k] = R;
-    }
-
-    {
-        const int left = max(x - a.bm_range, 0), right = min(x + a.bm_range, W - 8);
-        const int top = max(y - a.bm_range, 0), bottom = min(y + a.bm_range, H - 8);
-        for (int yy = top; yy <= bottom; ++yy) {
-            const float *row = centre + (long long)yy * S;
-            for (int xx = left; xx <= right; ++xx) insert8(er, xs, ys, zs, block_dist<M>(ref, row + xx, S), xx, yy, R);
+        for (int r = 0; r < 8; ++r) vo[r] = (lds_row(yy + r - ry0, LS) + X0) * 4;
+        float w[9][8];
+        lds_col<0>(w[0], lds, vo);
+        lds_col<4>(w[1], lds, vo);
+        lds_col<8>(w[2], lds, vo);
+        lds_col<12>(w[3], lds, vo);
+        lds_col<16>(w[4], lds, vo);
+        lds_col<20>(w[5], lds, vo);
+        lds_col<24>(w[6], lds, vo);
+        lds_col<28>(w[7], lds, vo);
+        const int x0 = x - Rg;
+        for (int tb = 0; tb <= span; tb += 9) {
+            const int xt = x0 + tb;
+            lds_step<M, 0>(w, lds, vo, W, xt + 0, row_ok, ybits, R, L);
+            if (tb + 1 > span) break;
+            lds_step<M, 1>(w, lds, vo, W, xt + 1, row_ok, ybits, R, L);
+            if (tb + 2 > span) break;
+            lds_step<M, 2>(w, lds, vo, W, xt + 2, row_ok, ybits, R, L);
+            if (tb + 3 > span) break;
+            lds_step<M, 3>(w, lds, vo, W, xt + 3, row_ok, ybits, R, L);
+            if (tb + 4 > span) break;
+            lds_step<M, 4>(w, lds, vo, W, xt + 4, row_ok, ybits, R, L);
+            if (tb + 5 > span) break;
+            lds_step<M, 5>(w, lds, vo, W, xt + 5, row_ok, ybits, R, L);
+            if (tb + 6 > span) break;
+            lds_step<M, 6>(w, lds, vo, W, xt + 6, row_ok, ybits, R, L);
+            if (tb + 7 > span) break;
+            lds_step<M, 7>(w, lds, vo, W, xt + 7, row_ok, ybits, R, L);
+            if (tb + 8 > span) break;
+            lds_step<M, 8>(w, lds, vo, W, xt + 8, row_ok, ybits, R, L);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) vo[r] += 36;
         }
     }
+
+    const unsigned self = pack_xy(x, y);
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) found |= L.pos[k] == self;
+    if (!found) {
+#pragma unroll
+        for (int k = 7; k > 0; --k) {
+            L.er[k] = L.er[k - 1];
+            L.pos[k] = L.pos[k - 1];
+        }
+        L.er[0] = 0.0f;
+        L.pos[0] = self;
+    }
+    if (!valid) return;
+    int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
+}
+
+template <int M, bool TEMPORAL>
+__global__ __launch_bounds__(64) void k_match(MatchArgs a) {
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
+    const bool valid = bx < a.nbx && by < a.nby;
+    const int W = a.width, H = a.height, S = a.stride, Rt = TEMPORAL ? a.radius : 0;
+    const int x = min(min(bx, a.nbx - 1) * a.block_step, W - 8);
+    const int y = min(min(by, a.nby - 1) * a.block_step, H - 8);
+    const float *centre = a.img + (long long)Rt * a.ts;
+    const PlaneBuf pb = make_plane_buf(a.img, (unsigned)(a.ts * (2 * Rt + 1) * 4));
+
+    RefPatch R;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) R.a[r * 8 + c] = centre[(long long)(y + r) * S + x + c];
+    ref_terms<M>(R);
+
+    List8 L;
+    list_init<TEMPORAL>(L, Rt);
+    scan_window<M, TEMPORAL>(pb, (long long)Rt * a.ts, S, W, H, R, x, y, a.bm_range, Rt, L);
 
     if constexpr (TEMPORAL) {
-        int cx[8], cy[8];
+        // predictive search (src/bm3d.cu:663-756): around the first ps_num entries of the
+        // previous frame's list, starting from the centre frame's list in each direction
+        unsigned cpos[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            cx[k] = xs[k];
-            cy[k] = ys[k];
-        }
+        for (int k = 0; k < 8; ++k) cpos[k] = L.pos[k];
         for (int dir = -1; dir <= 1; dir += 2) {
-            int lx[8], ly[8];
+            unsigned lpos[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                lx[k] = cx[k];
-                ly[k] = cy[k];
-            }
-            for (int t = 1; t <= R; ++t) {
-                const int tz = R + dir * t;
-                const float *frame = a.img + (long long)tz * a.ts;
-                float fe[8];
-                int fx[8], fy[8], fz[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    fe[k] = FLT_MAX;
-                    fx[k] = 0;
-                    fy[k] = 0;
-                    fz[k] = tz;
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    if (i < a.ps_num) {
-                        const int left = max(lx[i] - a.ps_range, 0), right = min(lx[i] + a.ps_range, W - 8);
-                        const int top = max(ly[i] - a.ps_range, 0), bottom = min(ly[i] + a.ps_range, H - 8);
-                        for (int yy = top; yy <= bottom; ++yy) {
-                            const float *row = frame + (long long)yy * S;
-                            for (int xx = left; xx <= right; ++xx)
-                                insert8(fe, fx, fy, fz, block_dist<M>(ref, row + xx, S), xx, yy, tz);
-                        }
-                    }
-                }
+            for (int k = 0; k < 8; ++k) lpos[k] = cpos[k];
+            for (int t = 1; t <= Rt; ++t) {
+                const int tz = Rt + dir * t;
+                List8 F;
+                list_init<false>(F, tz);
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
-                    if (i < a.ps_num) insert8(er, xs, ys, zs, fe[i], fx[i], fy[i], tz);
+                    if (i < a.ps_num)
+                        scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
+                                              a.ps_range, tz, F);
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    lx[k] = fx[k];
-                    ly[k] = fy[k];
-                }
+                for (int i = 0; i < 8; ++i)
+                    if (i < a.ps_num) insert8<true>(L, F.er[i], F.pos[i], tz);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) lpos[k] = F.pos[k];
             }
         }
     }
 
     // self-inclusion (src/bm3d.cu:758-788)
+    const unsigned self = pack_xy(x, y);
     bool found = false;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) found |= (xs[k] == x && ys[k] == y && (!TEMPORAL || zs[k] == R));
+    for (int k = 0; k < 8; ++k) found |= (L.pos[k] == self && (!TEMPORAL || L.zs[k] == Rt));
     if (!found) {
 #pragma unroll
         for (int k = 7; k > 0; --k) {
-            er[k] = er[k - 1];
-            xs[k] = xs[k - 1];
-            ys[k] = ys[k - 1];
-            zs[k] = zs[k - 1];
+            L.er[k] = L.er[k - 1];
+            L.pos[k] = L.pos[k - 1];
+            if constexpr (TEMPORAL) L.zs[k] = L.zs[k - 1];
         }
-        er[0] = 0.0f;
-        xs[0] = x;
-        ys[0] = y;
-        zs[0] = R;
+        L.er[0] = 0.0f;
+        L.pos[0] = self;
+        if constexpr (TEMPORAL) L.zs[0] = Rt;
     }
-    int4 *out = a.matches + (long long)gid * 8;
+    if (!valid) return;
+    int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) out[k] = make_int4(xs[k], ys[k], zs[k], __float_as_int(er[k]));
+    for (int k = 0; k < 8; ++k)
+        out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), TEMPORAL ? L.zs[k] : 0, __float_as_int(L.er[k]));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -701,6 +926,9 @@ bool kparams_ok(const vsz_bm3d_kparams *p) {
     if (p->block_step < 1 || p->block_step > 8 || p->bm_range < 1) return false;
     if (p->radius < 0 || p->radius > 16 || p->ps_num < 1 || p->ps_num > 8 || p->ps_range < 1) return false;
     if (p->metric < 0 || p->metric > 4 || p->t2d < 0 || p->t2d > 3 || p->t1d < 0 || p->t1d > 3) return false;
+    // the matcher addresses a [TW][H][S] plane stack with 32-bit byte offsets and packs x, y in 16 bits
+    if ((long long)(2 * p->radius + 1) * p->height * p->stride * 4 >= (1LL << 31)) return false;
+    if (p->width > 65535 || p->height > 65535) return false;
     return true;
 }
 
@@ -725,10 +953,32 @@ Layout layout_of(const vsz_bm3d_kparams *p) {
     return L;
 }
 
+// LDS bytes the staged matcher needs for a full 16x16-block tile (0: use the buffer path).
+constexpr size_t kMatchLdsBudget = 80 * 1024;  // two 256-lane workgroups per CU
+
+size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride) {
+    const long long rw = 15LL * block_step + 2LL * bm_range + 9;
+    const long long rh = 15LL * block_step + 2LL * bm_range + 8;
+    const long long ls = (rw + 3 + 31) / 32 * 32;
+    const size_t bytes = (size_t)(rh * ls * 4);
+    if (lds_stride) *lds_stride = (int)ls;
+    return bytes <= kMatchLdsBudget ? bytes : 0;
+}
+
 template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s) {
-    const unsigned grid = (unsigned)((a.nbx * (long long)a.nby + 255) / 256);
-    if (temporal) hipLaunchKernelGGL((k_match<M, true>), dim3(grid), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_match<M, false>), dim3(grid), dim3(256), 0, s, a);
+    if (!temporal) {
+        MatchArgs b = a;
+        const size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride);
+        if (lds) {
+            b.tiles_x = (a.nbx + 15) / 16;
+            const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 15) / 16)), 1, 1);
+            hipLaunchKernelGGL((k_match_lds<M>), grid, dim3(256), lds, s, b);
+            return;
+        }
+    }
+    const dim3 grid((unsigned)(a.tiles_x * ((a.nby + 7) / 8)), 1, 1);
+    if (temporal) hipLaunchKernelGGL((k_match<M, true>), grid, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_match<M, false>), grid, dim3(64), 0, s, a);
 }
 
 void launch_match(const MatchArgs &a, int metric, bool temporal, hipStream_t s) {
@@ -863,6 +1113,7 @@ int vsz_bm3d_match(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_mat
     m.ps_num = p->ps_num;
     m.ps_range = p->ps_range;
     m.ts = L.ts;
+    m.tiles_x = (L.nbx + 7) / 8;
     launch_match(m, p->metric, p->radius > 0, (hipStream_t)stream);
     return check_launch("vsz_bm3d_match");
 }

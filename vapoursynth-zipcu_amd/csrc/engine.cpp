@@ -567,6 +567,9 @@ int create_bm3d(const vsz_bm3d_args *a, bool fused_req, std::unique_ptr<Bm3d> &o
     }
     if (d->chroma && (vi.width < 8 || vi.height < 8)) return fail("BM3D: every processed plane must be at least 8x8.");
     for (auto &e : d->entries) {
+        // kernel-layer limits: 16-bit match coordinates, 32-bit byte offsets into a plane stack
+        if (e.w > 65535 || e.h > 65535 || size_t(d->tw) * e.plane_extent * 4 >= (size_t(1) << 31))
+            return fail("BM3D: plane too large for this build (width/height <= 65535, window < 2 GiB).");
         if (e.src_elems >= (size_t(1) << 31) || e.res_elems >= (size_t(1) << 31))
             return fail("BM3D: frame/radius too large (a device region exceeds 2^31 samples).");
         if (size_t(e.w) * e.h * d->tw * 2 * 4 >= (size_t(1) << 32))

@@ -29,6 +29,8 @@ struct MatchArgs {
     int width, height, stride, nbx, nby;
     int block_step, bm_range, radius, ps_num, ps_range;
     long long ts;
+    int tiles_x;     // block tiles per row
+    int lds_stride;  // LDS-staged path: floats per staged row
 };
 
 struct FilterArgs {
