@@ -785,7 +785,7 @@ __global__ __launch_bounds__(256) void k_filter_tile(FilterArgs a) {
     const int ry0 = max(min(by0 * bs, H - 8) - R, 0);
     const int ry1 = min(min((by0 + nby_t - 1) * bs, H - 8) + R, H - 8) + 8;
     const int rw = rx1 - rx0, cells = rw * (ry1 - ry0);
-    const long long guard = 1ll << a.guard_bits;
+    const float guard_f = __builtin_ldexpf(1.0f, a.guard_bits);
     const float E = a.extractor;
 
     for (int i = threadIdx.x; i < cells; i += 256) cell_i[i] = 0;
@@ -815,9 +815,11 @@ __global__ __launch_bounds__(256) void k_filter_tile(FilterArgs a) {
                 for (int j = 0; j < 8; ++j) {
                     const float q = (w * v[i * 8 + j] + E) - E;
                     if constexpr (EXACT) {
-                        const long long f = to_fixed(q, a.extractor_exp);
-                        if (inside && f < guard && f > -guard) atomicAdd(cell_i + cell0 + j * rw, (int)f);
-                        else atomicAdd((unsigned long long *)a.acc + gbase + (long long)j * a.stride, (unsigned long long)f);
+                        // q * 2^(24-e) is an integer; below the guard it converts exactly to int32
+                        const float sq = __builtin_ldexpf(q, 24 - a.extractor_exp);
+                        if (inside && fabsf(sq) < guard_f) atomicAdd(cell_i + cell0 + j * rw, (int)sq);
+                        else atomicAdd((unsigned long long *)a.acc + gbase + (long long)j * a.stride,
+                                       (unsigned long long)to_fixed(q, a.extractor_exp));
                     } else {
                         if (inside) atomicAdd(cell_f + cell0 + j * rw, q);
                         else atomicAdd((float *)a.acc + gbase + (long long)j * a.stride, q);
