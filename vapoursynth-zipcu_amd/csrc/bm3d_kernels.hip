@@ -597,46 +597,63 @@ __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Collaborative filtering. 8 lanes per group (lane = patch column c), 8 groups per wave; each
-// lane holds v[i*8 + j] = value at (rank i, row j, its column). The horizontal pass needs rows,
-// so each rank goes through a per-wave LDS transpose buffer; a workgroup-scope fence between
-// the writes and the reads makes the wave's own LDS writes visible to its reads.
+// Collaborative filtering. 8 lanes per group, 8 groups per wave. A lane holds 64 values of its
+// group: in column layout v[i*8 + j] is (rank i, row j, column = lane & 7); in row layout
+// v[i*8 + c] is (rank i, row = lane & 7, column c). The vertical passes run in column layout,
+// the horizontal passes, the group axis, the shrinkage and the aggregation in row layout; the
+// switch is an 8x8 transpose per rank through a per-group LDS buffer. LDS instructions of one
+// wave execute in order, so the transposes only need compiler barriers, not waits.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void lds_wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+__device__ __forceinline__ void lds_order() {
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    asm volatile("" ::: "memory");
 }
 
-__device__ __forceinline__ void transpose8x8(float *v, float *buf, int c) {
+// Switches all 8 ranks between column and row layout (the transpose is its own inverse).
+__device__ __forceinline__ void transpose_ranks(float *v, float *buf, int l8) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) buf[j * 9 + c] = v[j];
-    lds_wave_sync();
+    for (int i = 0; i < 8; ++i) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = buf[c * 9 + j];
-    lds_wave_sync();
+        for (int j = 0; j < 8; ++j) buf[j * 9 + l8] = v[i * 8 + j];
+        lds_order();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[i * 8 + j] = buf[l8 * 9 + j];
+        lds_order();
+    }
 }
 
-template <int T2, int T1, bool FWD> __device__ __forceinline__ void xform3d(float *v, float *buf, int c) {
-    // 2-D: vertical pass, transpose, horizontal pass, transpose back (src/bm3d.cu:461-466).
+template <int K, bool FWD> __device__ __forceinline__ void xform_rows(float *v) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) xf8<T2, FWD>(v + i * 8);
+    for (int i = 0; i < 8; ++i) xf8<K, FWD>(v + i * 8);
+}
+
+template <int K, bool FWD> __device__ __forceinline__ void xform_group_axis(float *v) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) transpose8x8(v + i * 8, buf, c);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) xf8<T2, FWD>(v + i * 8);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) transpose8x8(v + i * 8, buf, c);
-    // 1-D along the group axis.
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int c = 0; c < 8; ++c) {
         float t[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) t[i] = v[i * 8 + j];
-        xf8<T1, FWD>(t);
+        for (int i = 0; i < 8; ++i) t[i] = v[i * 8 + c];
+        xf8<K, FWD>(t);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i * 8 + j] = t[i];
+        for (int i = 0; i < 8; ++i) v[i * 8 + c] = t[i];
     }
+}
+
+// Forward 3-D transform (src/bm3d.cu:461-466): column layout in, row layout out.
+template <int T2, int T1> __device__ __forceinline__ void forward3d(float *v, float *buf, int l8) {
+    xform_rows<T2, true>(v);  // vertical pass: along j
+    transpose_ranks(v, buf, l8);
+    xform_rows<T2, true>(v);  // horizontal pass: along c
+    xform_group_axis<T1, true>(v);
+}
+
+// Inverse (src/bm3d.cu:470-475): row layout in, row layout out.
+template <int T2, int T1> __device__ __forceinline__ void inverse3d(float *v, float *buf, int l8) {
+    transpose_ranks(v, buf, l8);
+    xform_rows<T2, false>(v);  // vertical
+    transpose_ranks(v, buf, l8);
+    xform_rows<T2, false>(v);  // horizontal
+    xform_group_axis<T1, false>(v);
 }
 
 __device__ __forceinline__ float bfly8(float x) {
@@ -652,7 +669,7 @@ __device__ __forceinline__ long long to_fixed(float q, int e) {
     return (long long)s;
 }
 
-// Loads the group (and, in the final pass, the basic-estimate group) of column c.
+// Loads the group (and, in the final pass, the basic-estimate group) in column layout.
 template <bool FINAL>
 __device__ __forceinline__ void gather_group(const FilterArgs &a, int p, const int *mx, const int *my, const int *mz,
                                              int c, float *v, float *r) {
@@ -670,33 +687,38 @@ __device__ __forceinline__ void gather_group(const FilterArgs &a, int p, const i
 }
 
 // 3-D transform, shrinkage (src/bm3d.cu:416-546) and inverse; returns the adaptive weight.
+// Column layout in, row layout out. `r` is clobbered in the final pass.
 template <bool FINAL, int T2, int T1>
-__device__ __forceinline__ float filter_group(float *v, float *r, float sigma, float *buf, int c) {
-    xform3d<T2, T1, true>(v, buf, c);
+__device__ __forceinline__ float filter_group(float *v, float *r, float sigma, float *buf, int l8) {
+    forward3d<T2, T1>(v, buf, l8);
     float k = 0.0f;
     if constexpr (FINAL) {
-        xform3d<T2, T1, true>(r, buf, c);
+        forward3d<T2, T1>(r, buf, l8);
         const float s2 = sigma * sigma;
 #pragma unroll
         for (int f = 0; f < 64; ++f) {
             const float r2 = r[f] * r[f];
             float coeff = __fdiv_rn(r2, r2 + s2);
-            if (f == 0 && c == 0) coeff = 1.0f;
-            const float val = v[f] * coeff;
-            k = fmaf_(coeff, coeff, k);
-            v[f] = val * (1.0f / 4096.0f);
+            if (f == 0 && l8 == 0) coeff = 1.0f;  // 3-D DC: rank 0, row 0, column 0
+            v[f] = (v[f] * coeff) * (1.0f / 4096.0f);
+            r[f] = coeff;
         }
+        // k sums coeff^2 per column over (rank, row) in that order (src/bm3d.cu:491-504), so
+        // the coefficients go back to column layout for the accumulation.
+        transpose_ranks(r, buf, l8);
+#pragma unroll
+        for (int f = 0; f < 64; ++f) k = fmaf_(r[f], r[f], k);
     } else {
 #pragma unroll
         for (int f = 0; f < 64; ++f) {
-            const float thr = (f == 0 && c == 0) ? 0.0f : sigma;
+            const float thr = (f == 0 && l8 == 0) ? 0.0f : sigma;
             const bool keep = fabsf(v[f]) >= thr;
             k = k + (keep ? 1.0f : 0.0f);
             v[f] = keep ? v[f] * (1.0f / 4096.0f) : 0.0f;
         }
     }
     const float w = __fdiv_rn(1.0f, bfly8(k));
-    xform3d<T2, T1, false>(v, buf, c);
+    inverse3d<T2, T1>(v, buf, l8);
     return w;
 }
 
@@ -746,14 +768,14 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
         if (!valid) continue;
         const long long pb = (long long)p * a.acc_plane_elems;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const long long base = pb + (long long)mz[i] * 2 * a.ts + (long long)my[i] * a.stride + mx[i] + c;
+        for (int i = 0; i < 8; ++i) {  // row layout: this lane adds row (lane & 7) of patch i
+            const long long base = pb + (long long)mz[i] * 2 * a.ts + (long long)(my[i] + c) * a.stride + mx[i];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const float q = (w * v[i * 8 + j] + E) - E;
-                if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + base + (long long)j * a.stride,
+            for (int cc = 0; cc < 8; ++cc) {
+                const float q = (w * v[i * 8 + cc] + E) - E;
+                if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + base + cc,
                                                (unsigned long long)to_fixed(q, a.extractor_exp));
-                else atomicAdd((float *)a.acc + base + (long long)j * a.stride, q);
+                else atomicAdd((float *)a.acc + base + cc, q);
             }
         }
         if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
@@ -769,7 +791,7 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
 // is flushed once with 64-bit device atomics; integer addition commutes, so the result does
 // not depend on the order of any of these updates.
 template <bool FINAL, int T2, int T1, bool EXACT>
-__global__ __launch_bounds__(256) void k_filter_tile(FilterArgs a) {
+__global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
     extern __shared__ float smem[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 3, c = lane & 7;
     float *buf = smem + (wave * 8 + g) * 72;
@@ -807,22 +829,22 @@ __global__ __launch_bounds__(256) void k_filter_tile(FilterArgs a) {
             const float w = filter_group<FINAL, T2, T1>(v, r, a.sigma[p], buf, c);
             if (!valid) continue;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < 8; ++i) {  // row layout: this lane adds row (lane & 7) of patch i
                 const bool inside = mx[i] >= rx0 && mx[i] + 8 <= rx1 && my[i] >= ry0 && my[i] + 8 <= ry1;
-                const int cell0 = (my[i] - ry0) * rw + mx[i] - rx0 + c;
-                const long long gbase = pb + (long long)my[i] * a.stride + mx[i] + c;
+                const int cell0 = (my[i] + c - ry0) * rw + mx[i] - rx0;
+                const long long gbase = pb + (long long)(my[i] + c) * a.stride + mx[i];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const float q = (w * v[i * 8 + j] + E) - E;
+                for (int cc = 0; cc < 8; ++cc) {
+                    const float q = (w * v[i * 8 + cc] + E) - E;
                     if constexpr (EXACT) {
                         // q * 2^(24-e) is an integer; below the guard it converts exactly to int32
                         const float sq = __builtin_ldexpf(q, 24 - a.extractor_exp);
-                        if (inside && fabsf(sq) < guard_f) atomicAdd(cell_i + cell0 + j * rw, (int)sq);
-                        else atomicAdd((unsigned long long *)a.acc + gbase + (long long)j * a.stride,
+                        if (inside && fabsf(sq) < guard_f) atomicAdd(cell_i + cell0 + cc, (int)sq);
+                        else atomicAdd((unsigned long long *)a.acc + gbase + cc,
                                        (unsigned long long)to_fixed(q, a.extractor_exp));
                     } else {
-                        if (inside) atomicAdd(cell_f + cell0 + j * rw, q);
-                        else atomicAdd((float *)a.acc + gbase + (long long)j * a.stride, q);
+                        if (inside) atomicAdd(cell_f + cell0 + cc, q);
+                        else atomicAdd((float *)a.acc + gbase + cc, q);
                     }
                 }
             }
