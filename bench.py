@@ -66,6 +66,28 @@ def cpu_baseline(nz, strip_rows):
                       f"({dt:.1f} s wall, {threads} threads)"}
 
 
+def profile_target(args):
+    import torch
+    from vszipcu import device, synth
+    if args.profile_target == "copy":
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("bench: copied 3 x 256 MiB", flush=True)
+        return
+    _, nz = synth.yuv444_frame(W, H, SIGMA, seed=2)
+    kp = device.kparams(W, H, sigma=device.scaled_sigma(SIGMA, False), **PARAMS)
+    img = torch.from_numpy(nz[0][None]).cuda()
+    nbx, nby = device.num_blocks(kp)
+    m = torch.zeros((nby, nbx, 8, 4), dtype=torch.int32, device="cuda")
+    for _ in range(args.steps):
+        device.match(kp, img, m)
+    torch.cuda.synchronize()
+    print(f"bench: {args.steps} matching launches", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +95,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-strip-rows", type=int, default=272)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-target", choices=["step", "match", "copy"], default="step",
+                    help="match: only the basic-pass matching of the bench frame, --steps launches (for "
+                         "rocprofv3 --pmc runs); copy: a 256 MiB device copy, 3 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -83,6 +108,8 @@ def main():
     from vszipcu import device, synth
 
     assert torch.cuda.is_available(), "bench needs a GPU"
+    if args.profile_target != "step":
+        return profile_target(args)
     print(f"bench: workload 1920x1080 YUV444 two-pass BM3D, steps={args.steps} warmup={args.warmup}", file=sys.stderr,
           flush=True)
     clean, nz = synth.yuv444_frame(W, H, SIGMA, seed=2)

@@ -833,16 +833,31 @@ __global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
                 const bool inside = mx[i] >= rx0 && mx[i] + 8 <= rx1 && my[i] >= ry0 && my[i] + 8 <= ry1;
                 const int cell0 = (my[i] + c - ry0) * rw + mx[i] - rx0;
                 const long long gbase = pb + (long long)(my[i] + c) * a.stride + mx[i];
+                if constexpr (EXACT) {
+                    // q * 2^(24-e) is an integer; below the guard it converts exactly to int32.
+                    // Cells take 0 for contributions that must go to the device accumulator.
+                    const int cbase = inside ? cell0 : 0;
+                    bool spill = false;
+                    float sq[8];
 #pragma unroll
-                for (int cc = 0; cc < 8; ++cc) {
-                    const float q = (w * v[i * 8 + cc] + E) - E;
-                    if constexpr (EXACT) {
-                        // q * 2^(24-e) is an integer; below the guard it converts exactly to int32
-                        const float sq = __builtin_ldexpf(q, 24 - a.extractor_exp);
-                        if (inside && fabsf(sq) < guard_f) atomicAdd(cell_i + cell0 + cc, (int)sq);
-                        else atomicAdd((unsigned long long *)a.acc + gbase + cc,
-                                       (unsigned long long)to_fixed(q, a.extractor_exp));
-                    } else {
+                    for (int cc = 0; cc < 8; ++cc) {
+                        sq[cc] = __builtin_ldexpf((w * v[i * 8 + cc] + E) - E, 24 - a.extractor_exp);
+                        const bool ok = inside && fabsf(sq[cc]) < guard_f;
+                        spill |= !ok;
+                        atomicAdd(cell_i + cbase + cc, ok ? (int)sq[cc] : 0);
+                    }
+                    if (spill) {
+#pragma unroll
+                        for (int cc = 0; cc < 8; ++cc)
+                            if (!(inside && fabsf(sq[cc]) < guard_f))
+                                atomicAdd((unsigned long long *)a.acc + gbase + cc,
+                                          (unsigned long long)to_fixed(__builtin_ldexpf(sq[cc], a.extractor_exp - 24),
+                                                                       a.extractor_exp));
+                    }
+                } else {
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) {
+                        const float q = (w * v[i * 8 + cc] + E) - E;
                         if (inside) atomicAdd(cell_f + cell0 + cc, q);
                         else atomicAdd((float *)a.acc + gbase + cc, q);
                     }
