@@ -164,6 +164,12 @@ int vsz_filter_output_info(const vsz_filter *f, vsz_output_info *info);
 int vsz_filter_get_frame(vsz_filter *f, int n, vsz_get_plane_fn get, void *user, void *const dst[3],
                          const ptrdiff_t dst_stride[3], char *err, size_t errlen);
 
+/* Frames of input `which` (as in vsz_get_plane_fn) that vsz_filter_get_frame(f, n, ...) may
+ * pull: the inclusive range [*first, *last], already clamped to the clip; *first > *last when
+ * the filter does not read that input. These are the frames the reference requests in the
+ * arInitial activation (src/bm3d.zig:643-651, fused 2r window; :1328-1332 for VAggregate). */
+int vsz_filter_frame_range(const vsz_filter *f, int which, int n, int *first, int *last);
+
 void vsz_filter_free(vsz_filter *f);
 
 #ifdef __cplusplus

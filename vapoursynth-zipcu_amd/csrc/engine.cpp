@@ -343,6 +343,8 @@ struct vsz_filter {
     vsz_output_info out{};
     virtual int get_frame(int n, vsz_get_plane_fn get, void *user, void *const dst[3], const ptrdiff_t dst_stride[3]) = 0;
     virtual const char *frame_error() const { return "BM3D: process frame failed."; }
+    // Inclusive range of frames of input `which` that get_frame(n) may pull; lo > hi if none.
+    virtual void frame_range(int which, int n, int &lo, int &hi) const = 0;
 };
 
 namespace {
@@ -391,6 +393,16 @@ class Bm3d final : public vsz_filter {
 
     bool normal_out() const { return radius == 0 || fused; }
     const char *frame_error() const override { return v2_error.c_str(); }
+    // src/bm3d.zig:643-651: n-r..n+r (n-2r..n+2r when fused) of clip and ref
+    void frame_range(int which, int n, int &lo, int &hi) const override {
+        const int req = fused ? 2 * radius : radius;
+        lo = 0;
+        hi = -1;
+        if (which == 0 || (which == 1 && n_clips == 2)) {
+            lo = std::max(n - req, 0);
+            hi = std::min(n + req, vi.num_frames - 1);
+        }
+    }
 
     int init_device(char *err, size_t errlen);
     int get_frame(int n, vsz_get_plane_fn get, void *user, void *const dst[3], const ptrdiff_t dst_stride[3]) override;
@@ -896,6 +908,12 @@ class VAggregate final : public vsz_filter {
     bool process[3] = {false, false, false};
 
     const char *frame_error() const override { return "VAggregate: process frame failed."; }
+    // src/bm3d.zig:1328-1332: stacked frames n-r..n+r, src frame n
+    void frame_range(int which, int n, int &lo, int &hi) const override {
+        lo = which == 0 ? std::max(n - radius, 0) : n;
+        hi = which == 0 ? std::min(n + radius, src_vi.num_frames - 1) : n;
+        if (which > 1) hi = lo - 1;
+    }
 
     // which 0 = stacked clip, 1 = src
     int get_frame(int n, vsz_get_plane_fn get, void *user, void *const dst[3], const ptrdiff_t dst_stride[3]) override {
@@ -952,6 +970,16 @@ class Composed final : public vsz_filter {
     std::unique_ptr<VAggregate> vagg;
 
     const char *frame_error() const override { return "BM3D: process frame failed."; }
+    // the BM3D frames n-r..n+r of the aggregation window each read their own n-r..n+r window
+    void frame_range(int which, int n, int &lo, int &hi) const override {
+        const int r = vagg->radius;
+        lo = 0;
+        hi = -1;
+        if (which == 0 || (which == 1 && bm3d->n_clips == 2)) {
+            lo = std::max(n - 2 * r, 0);
+            hi = std::min(n + 2 * r, vagg->src_vi.num_frames - 1);
+        }
+    }
 
     int get_frame(int n, vsz_get_plane_fn get, void *user, void *const dst[3], const ptrdiff_t dst_stride[3]) override {
         const int nf = vagg->src_vi.num_frames, r = vagg->radius, tw = 2 * r + 1;
@@ -1039,6 +1067,10 @@ class Composed final : public vsz_filter {
 class Passthrough final : public vsz_filter {
   public:
     vsz_clip_info vi{};
+    void frame_range(int which, int n, int &lo, int &hi) const override {
+        lo = n;
+        hi = which == 0 ? n : n - 1;
+    }
     int get_frame(int n, vsz_get_plane_fn get, void *user, void *const dst[3], const ptrdiff_t dst_stride[3]) override {
         for (int p = 0; p < std::min(vi.num_planes, 3); ++p) {
             ptrdiff_t st = 0;
@@ -1175,6 +1207,15 @@ int vsz_filter_get_frame(vsz_filter *f, int n, vsz_get_plane_fn get, void *user,
         std::fprintf(stderr, "vszipcu: %s (%s)\n", f->frame_error(), detail.c_str());
         return set_err(err, errlen, rc, f->frame_error());
     }
+    return VSZ_OK;
+}
+
+int vsz_filter_frame_range(const vsz_filter *f, int which, int n, int *first, int *last) {
+    if (!f || !first || !last) return vsz_fail(VSZ_EINVAL, "null argument");
+    int lo = 0, hi = -1;
+    f->frame_range(which, n, lo, hi);
+    *first = lo;
+    *last = hi;
     return VSZ_OK;
 }
 

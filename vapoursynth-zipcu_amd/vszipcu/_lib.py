@@ -89,6 +89,7 @@ PROTOTYPES = {
     "vsz_filter_output_info": (C.c_int, [C.c_void_p, C.POINTER(OutputInfo)]),
     "vsz_filter_get_frame": (C.c_int, [C.c_void_p, C.c_int, GET_PLANE_FN, C.c_void_p, C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_ssize_t), C.c_char_p, C.c_size_t]),
+    "vsz_filter_frame_range": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vsz_filter_free": (None, [C.c_void_p]),
 }
 
