@@ -9,7 +9,12 @@ filtering, aggregation, division) over the frame. value = Mpix/s = 1920*1080 / s
 Roofline: the dominant kernel's algorithmic work is counted from the reference geometry:
 SSD_FLOP = 192 per candidate block evaluation, summed over the clipped windows of every
 reference block of both passes. Its duration comes from HIP events on the launch stream.
-CPU baseline: the FP32 oracle (multithreaded C++) on a bounded strip of the same workload.
+step_roofline: the whole step's lower bound T_min = (SSD_FLOP + TF_FLOP) / P_fp32, with
+TF_FLOP = 128 per 8-point transform (384 per group and plane for the hard pass, 576 for Wiener),
+against the measured step time.
+end_to_end: the same two passes through the filter layer on host frames (upload, kernels,
+download, 4 streams), in Mpix/s; reported beside `value`, which is device-resident.
+CPU baseline: the FP32 oracle (multithreaded C++) on the full frame, plus one thread on a strip.
 """
 import argparse
 import json
@@ -45,25 +50,48 @@ def ssd_candidates(width, height, step, rng):
     return int(nx.sum()) * int(ny.sum())
 
 
-def cpu_baseline(nz, strip_rows):
+def _oracle_two_pass(nz, rows, threads):
     import oracle
     from vszipcu import device
-    threads = min(os.cpu_count() or 1, 64)
-    strip = np.ascontiguousarray(nz[:, :strip_rows])
-    sb = device.scaled_sigma(SIGMA, False)
-    sf = device.scaled_sigma(SIGMA, True)
+    strip = np.ascontiguousarray(nz[:, :rows])
     common = {k: v for k, v in PARAMS.items()}
-    pb = oracle.make_params(W, strip_rows, sigma=tuple(sb), threads=threads, **common)
-    pf = oracle.make_params(W, strip_rows, sigma=tuple(sf), final=True, threads=threads, **common)
+    pb = oracle.make_params(W, rows, sigma=tuple(device.scaled_sigma(SIGMA, False)), threads=threads, **common)
+    pf = oracle.make_params(W, rows, sigma=tuple(device.scaled_sigma(SIGMA, True)), final=True, threads=threads,
+                            **common)
     t0 = time.perf_counter()
     res, _, _ = oracle.bm3d(pb, strip[None, :, None])
     basic = oracle.aggregate(res[:, 0], W)
     res2, _, _ = oracle.bm3d(pf, np.stack([basic, strip])[:, :, None])
     oracle.aggregate(res2[:, 0], W)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(nz, strip_rows):
+    threads = min(os.cpu_count() or 1, 64)
+    dt = _oracle_two_pass(nz, H, threads)
+    dt1 = _oracle_two_pass(nz, strip_rows, 1)
+    return {"value": round(W * H / dt / 1e6, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 oracle, both passes over the whole {W}x{H} frame ({dt:.1f} s wall, {threads} threads)",
+            "single_thread": {"value": round(W * strip_rows / dt1 / 1e6, 4), "unit": "Mpix/s",
+                              "sample": f"1 thread, both passes on a {W}x{strip_rows} strip ({dt1:.1f} s)"}}
+
+
+def end_to_end(nz, frames=12, workers=4):
+    """Two passes through the filter layer on host frames: upload, kernels, download."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from vszipcu import YUV444PS, Clip, core
+    clip = Clip.from_frames([list(nz)] * frames, YUV444PS)
+    kw = dict(sigma=SIGMA, block_step=4, bm_range=16, ps_num=2, ps_range=4, chroma=True, extractor_exp=3,
+              num_streams=workers)
+    basic = core.vszipcu.BM3D(clip, **kw)
+    final = core.vszipcu.BM3D(clip, ref=basic, **kw)
+    final.get_frame(0)  # warm: streams, pinned staging
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda n: final.get_frame(n), range(1, frames)))
     dt = time.perf_counter() - t0
-    return {"value": round(W * strip_rows / dt / 1e6, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
-            "sample": f"FP32 oracle, both passes on a {W}x{strip_rows} strip of the same frame "
-                      f"({dt:.1f} s wall, {threads} threads)"}
+    return round(W * H * (frames - 1) / dt / 1e6, 3)
 
 
 def profile_target(args):
@@ -93,7 +121,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-strip-rows", type=int, default=272)
+    ap.add_argument("--cpu-strip-rows", type=int, default=136, help="rows of the single-thread CPU sample")
+    ap.add_argument("--no-end-to-end", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-target", choices=["step", "match", "copy"], default="step",
                     help="match: only the basic-pass matching of the bench frame, --steps launches (for "
@@ -170,6 +199,15 @@ def main():
     out = d_final.cpu().numpy()
     psnr_y = synth.psnr(out[0], clean[0])
 
+    nbx, nby = device.num_blocks(kp_b)
+    tf_flop = 128.0 * nbx * nby * 3 * (384 + 576)
+    t_min = (flop + tf_flop) / (PEAK_FP32_TFLOPS * 1e12)
+    e2e = None
+    if not args.no_end_to_end:
+        print("bench: end-to-end through the filter layer ...", file=sys.stderr, flush=True)
+        e2e = end_to_end(nz)
+        print(f"bench: end-to-end {e2e} Mpix/s", file=sys.stderr, flush=True)
+
     result = {
         "metric": METRIC,
         "value": round(mpix, 3),
@@ -189,6 +227,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": "k_match (SSD block matching, both passes)"},
+        "step_roofline": {"ssd_gflop": round(flop / 1e9, 3), "tf_gflop": round(tf_flop / 1e9, 3),
+                          "t_min_ms": round(t_min * 1e3, 4), "frac": round(t_min / dt, 4)},
+        "end_to_end_mpix_s": e2e,
     }
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)

@@ -674,7 +674,37 @@ int run_bm3d(const OracleParams &p, const T *src, T *res, int32_t *matches_out, 
             }
         });
 
-        // Sequential accumulation in reference-block raster order.
+        if (exact) {
+            // Integer sums do not depend on order: each thread owns a band of image rows and
+            // adds every contribution that lands in it.
+            const int bands = std::min(threads, H);
+            parallel_for(bands, bands, [&](int t) {
+                const int y0 = int(long(H) * t / bands), y1 = int(long(H) * (t + 1) / bands);
+                for (int bi = 0; bi < nblk; ++bi) {
+                    const Cand<T> *l = &lists[long(bi) * 8];
+                    for (int pl = 0; pl < P; ++pl) {
+                        const GroupOut<T> &o = outs[long(bi) * P + pl];
+                        if (!o.active) continue;
+                        const int64_t wq = to_fixed(float(o.weight), p.extractor_exp);
+                        const long pbase = long(pl) * TW * 2 * TS;
+                        for (int i = 0; i < 8; ++i) {
+                            const long base = pbase + long(l[i].z) * 2 * TS + long(l[i].y) * S + l[i].x;
+                            for (int j = 0; j < 8; ++j) {
+                                const int y = l[i].y + j;
+                                if (y < y0 || y >= y1) continue;
+                                for (int c = 0; c < 8; ++c) {
+                                    const long a = base + long(j) * S + c;
+                                    acc_fixed[a] += to_fixed(float(o.vals[i][j][c]), p.extractor_exp);
+                                    acc_fixed[a + TS] += wq;
+                                }
+                            }
+                        }
+                    }
+                }
+            });
+            continue;
+        }
+        // Float sums: sequential, in reference-block raster order.
         for (int bi = 0; bi < nblk; ++bi) {
             const Cand<T> *l = &lists[long(bi) * 8];
             for (int pl = 0; pl < P; ++pl) {
@@ -686,13 +716,8 @@ int run_bm3d(const OracleParams &p, const T *src, T *res, int32_t *matches_out, 
                     for (int j = 0; j < 8; ++j)
                         for (int c = 0; c < 8; ++c) {
                             const long a = base + long(j) * S + c;
-                            if (exact) {
-                                acc_fixed[a] += to_fixed(float(o.vals[i][j][c]), p.extractor_exp);
-                                acc_fixed[a + TS] += to_fixed(float(o.weight), p.extractor_exp);
-                            } else {
-                                res[a] = res[a] + o.vals[i][j][c];
-                                res[a + TS] = res[a + TS] + o.weight;
-                            }
+                            res[a] = res[a] + o.vals[i][j][c];
+                            res[a + TS] = res[a + TS] + o.weight;
                         }
                 }
             }
