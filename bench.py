@@ -199,6 +199,27 @@ def main():
     out = d_final.cpu().numpy()
     psnr_y = synth.psnr(out[0], clean[0])
 
+    # the same two passes with the DCT stages as compensated split-BF16 MFMA products
+    kp_b16 = device.kparams(W, H, sigma=device.scaled_sigma(SIGMA, False), precision="split-bf16", **PARAMS)
+    kp_f16 = device.kparams(W, H, sigma=device.scaled_sigma(SIGMA, True), final=True, precision="split-bf16",
+                            **PARAMS)
+
+    def step16():
+        device.run(kp_b16, d_noisy, None, work_b, d_basic.view(3, H, W), 0, stream)
+        device.run(kp_f16, d_basic, d_noisy, work_f, d_final, 0, stream)
+
+    for _ in range(args.warmup):
+        step16()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step16()
+    torch.cuda.synchronize()
+    dt16 = (time.perf_counter() - t0) / args.steps
+    psnr16 = synth.psnr(d_final.cpu().numpy()[0], clean[0])
+    print(f"bench: split-bf16 transforms {dt16 * 1e3:.3f} ms/step, {W * H / dt16 / 1e6:.2f} Mpix/s, "
+          f"PSNR {psnr16:.4f} dB (plain {psnr_y:.4f})", file=sys.stderr, flush=True)
+
     nbx, nby = device.num_blocks(kp_b)
     tf_flop = 128.0 * nbx * nby * 3 * (384 + 576)
     t_min = (flop + tf_flop) / (PEAK_FP32_TFLOPS * 1e12)
@@ -230,6 +251,10 @@ def main():
         "step_roofline": {"ssd_gflop": round(flop / 1e9, 3), "tf_gflop": round(tf_flop / 1e9, 3),
                           "t_min_ms": round(t_min * 1e3, 4), "frac": round(t_min / dt, 4)},
         "end_to_end_mpix_s": e2e,
+        "forms": {"split_bf16_dct": {"value": round(W * H / dt16 / 1e6, 3), "ms_per_step": round(dt16 * 1e3, 4),
+                                     "psnr_y_db": round(psnr16, 4), "psnr_delta_db": round(psnr16 - psnr_y, 4),
+                                     "note": "DCT stages as compensated split-BF16 MFMA, FP32 accumulation; "
+                                             "matching FP32"}},
     }
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)

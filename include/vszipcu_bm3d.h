@@ -58,6 +58,9 @@ typedef struct vsz_bm3d_kparams {
     int metric;                        /* 0 ssd, 1 sad, 2 zssd, 3 zsad, 4 ssd/norm */
     int t2d, t1d;                      /* 0 dct, 1 haar, 2 wht, 3 bior1.5 */
     unsigned proc_mask;                /* planes the final division writes (src/bm3d.cu:910-914) */
+    int transform_precision;           /* 0: FP32 transforms with the reference's op order (default);
+                                          1: DCT stages as compensated split-BF16 MFMA products with
+                                          FP32 accumulation (dct/dct only; matching stays FP32) */
 } vsz_bm3d_kparams;
 
 /* Reference blocks per axis (src/bm3d.cu:588-595): ceil((N-8)/block_step)+1. */

@@ -622,18 +622,102 @@ __device__ __forceinline__ void transpose_ranks(float *v, float *buf, int l8) {
     }
 }
 
-template <int K, bool FWD> __device__ __forceinline__ void xform_rows(float *v) {
+// ---- compensated split-BF16 MFMA form of the DCT stages ---------------------------------------
+// Transform kind kDctMfma is the DCT computed as y = M x with M = 4*C (C the orthonormal DCT-II;
+// its transpose for the inverse), which is what the reference codelets compute up to rounding.
+// Each FP32 operand is split into bf16 hi + lo (hi = RNE(x), lo = RNE(x - hi)); the product is
+// Mh*xh + Mh*xl + Ml*xh (the lo*lo term is below 2^-16 relative), accumulated in FP32 by the
+// 4x4x4 16-block bf16 MFMA. In that instruction a lane owns one column of B and D per block,
+// so a lane's 8-vector stays in the lane: B = the vector's 4 inputs of a k-block, D = 4 outputs,
+// and the lane supplies row (lane & 3) of the 4x4 sub-block of M as its A operand.
+constexpr int kDctMfma = 4;
+
+typedef short vsz_s4 __attribute__((ext_vector_type(4)));
+typedef float vsz_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 vsz_b4 __attribute__((ext_vector_type(4)));
+
+__constant__ float kDct8x4[64] = {  // 4*C[u][x], row u = frequency
+    1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f,
+    1.961570501e+00f, 1.662939191e+00f, 1.111140490e+00f, 3.901806474e-01f, -3.901806474e-01f, -1.111140490e+00f, -1.662939191e+00f, -1.961570501e+00f,
+    1.847759008e+00f, 7.653668523e-01f, -7.653668523e-01f, -1.847759008e+00f, -1.847759008e+00f, -7.653668523e-01f, 7.653668523e-01f, 1.847759008e+00f,
+    1.662939191e+00f, -3.901806474e-01f, -1.961570501e+00f, -1.111140490e+00f, 1.111140490e+00f, 1.961570501e+00f, 3.901806474e-01f, -1.662939191e+00f,
+    1.414213538e+00f, -1.414213538e+00f, -1.414213538e+00f, 1.414213538e+00f, 1.414213538e+00f, -1.414213538e+00f, -1.414213538e+00f, 1.414213538e+00f,
+    1.111140490e+00f, -1.961570501e+00f, 3.901806474e-01f, 1.662939191e+00f, -1.662939191e+00f, -3.901806474e-01f, 1.961570501e+00f, -1.111140490e+00f,
+    7.653668523e-01f, -1.847759008e+00f, 1.847759008e+00f, -7.653668523e-01f, -7.653668523e-01f, 1.847759008e+00f, -1.847759008e+00f, 7.653668523e-01f,
+    3.901806474e-01f, -1.111140490e+00f, 1.662939191e+00f, -1.961570501e+00f, 1.961570501e+00f, -1.662939191e+00f, 1.111140490e+00f, -3.901806474e-01f,
+};
+
+struct SplitOp {
+    vsz_s4 hi, lo;
+};
+
+__device__ __forceinline__ SplitOp split4(float a, float b, float c, float d) {
+    const vsz_f4 x = {a, b, c, d};
+    const vsz_b4 h = __builtin_convertvector(x, vsz_b4);
+    const vsz_f4 r = x - __builtin_convertvector(h, vsz_f4);  // exact: hi carries x's top 8 bits
+    const vsz_b4 l = __builtin_convertvector(r, vsz_b4);
+    return SplitOp{__builtin_bit_cast(vsz_s4, h), __builtin_bit_cast(vsz_s4, l)};
+}
+
+// A operands of M (FWD) or M^T: sub-block (q, kb) = rows 4q.., columns 4kb.., this lane's row.
+template <bool FWD> __device__ __forceinline__ void load_dct_ops(SplitOp (&A)[2][2]) {
+    const int row = threadIdx.x & 3;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) xf8<K, FWD>(v + i * 8);
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            float e[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                e[k] = FWD ? kDct8x4[(4 * q + row) * 8 + 4 * kb + k] : kDct8x4[(4 * kb + k) * 8 + 4 * q + row];
+            A[q][kb] = split4(e[0], e[1], e[2], e[3]);
+        }
+}
+
+__device__ __forceinline__ vsz_f4 mf4(vsz_s4 a, vsz_s4 b, vsz_f4 c) {
+    return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// y = M x on this lane's 8-vector (every lane of the wave takes part).
+__device__ __forceinline__ void dct8_mfma(float *x, const SplitOp (&A)[2][2]) {
+    const SplitOp b0 = split4(x[0], x[1], x[2], x[3]);
+    const SplitOp b1 = split4(x[4], x[5], x[6], x[7]);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        vsz_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mf4(A[q][0].lo, b0.hi, acc);  // small cross terms first
+        acc = mf4(A[q][0].hi, b0.lo, acc);
+        acc = mf4(A[q][1].lo, b1.hi, acc);
+        acc = mf4(A[q][1].hi, b1.lo, acc);
+        acc = mf4(A[q][0].hi, b0.hi, acc);
+        acc = mf4(A[q][1].hi, b1.hi, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[4 * q + r] = acc[r];
+    }
+}
+
+template <int K, bool FWD> __device__ __forceinline__ void xform_rows(float *v) {
+    if constexpr (K == kDctMfma) {
+        SplitOp A[2][2];
+        load_dct_ops<FWD>(A);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dct8_mfma(v + i * 8, A);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) xf8<K, FWD>(v + i * 8);
+    }
 }
 
 template <int K, bool FWD> __device__ __forceinline__ void xform_group_axis(float *v) {
+    SplitOp A[2][2];
+    if constexpr (K == kDctMfma) load_dct_ops<FWD>(A);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         float t[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) t[i] = v[i * 8 + c];
-        xf8<K, FWD>(t);
+        if constexpr (K == kDctMfma) dct8_mfma(t, A);
+        else xf8<K, FWD>(t);
 #pragma unroll
         for (int i = 0; i < 8; ++i) v[i * 8 + c] = t[i];
     }
@@ -965,6 +1049,9 @@ bool kparams_ok(const vsz_bm3d_kparams *p) {
     if (p->block_step < 1 || p->block_step > 8 || p->bm_range < 1) return false;
     if (p->radius < 0 || p->radius > 16 || p->ps_num < 1 || p->ps_num > 8 || p->ps_range < 1) return false;
     if (p->metric < 0 || p->metric > 4 || p->t2d < 0 || p->t2d > 3 || p->t1d < 0 || p->t1d > 3) return false;
+    // the split-BF16 MFMA form exists for the DCT pair only
+    if (p->transform_precision < 0 || p->transform_precision > 1) return false;
+    if (p->transform_precision == 1 && (p->t2d != 0 || p->t1d != 0)) return false;
     // the matcher addresses a [TW][H][S] plane stack with 32-bit byte offsets and packs x, y in 16 bits
     if ((long long)(2 * p->radius + 1) * p->height * p->stride * 4 >= (1LL << 31)) return false;
     if (p->width > 65535 || p->height > 65535) return false;
@@ -1207,8 +1294,14 @@ int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d
     f.tbx = tp.tbx;
     f.tby = tp.tby;
     f.guard_bits = tp.guard_bits;
-    if (p->final_pass) launch_filter_1<true>(f, tp, p->t2d, p->t1d, s);
-    else launch_filter_1<false>(f, tp, p->t2d, p->t1d, s);
+    if (p->transform_precision == 1) {
+        if (p->final_pass) launch_filter_3<true, kDctMfma, kDctMfma>(f, tp, s);
+        else launch_filter_3<false, kDctMfma, kDctMfma>(f, tp, s);
+    } else if (p->final_pass) {
+        launch_filter_1<true>(f, tp, p->t2d, p->t1d, s);
+    } else {
+        launch_filter_1<false>(f, tp, p->t2d, p->t1d, s);
+    }
     rc = check_launch("vsz_bm3d_run: filter");
     if (rc) return rc;
 

@@ -25,6 +25,7 @@ class KParams(C.Structure):
         ("extractor_exp", C.c_int),
         ("metric", C.c_int), ("t2d", C.c_int), ("t1d", C.c_int),
         ("proc_mask", C.c_uint),
+        ("transform_precision", C.c_int),
     ]
 
 

@@ -34,8 +34,9 @@ def scaled_sigma(sigma255, final: bool):
 
 def kparams(width, height, stride=None, *, block_step=8, bm_range=9, radius=0, ps_num=2, ps_range=4,
             chroma=False, final=False, sigma=(0.0, 0.0, 0.0), extractor_exp=0, metric="ssd", t2d="dct",
-            t1d="dct", proc_mask=0b111) -> _lib.KParams:
-    """Kernel-level parameters; `sigma` are the SCALED thresholds."""
+            t1d="dct", proc_mask=0b111, precision="fp32") -> _lib.KParams:
+    """Kernel-level parameters; `sigma` are the SCALED thresholds. precision "fp32" keeps the
+    reference arithmetic; "split-bf16" runs the DCT stages as compensated BF16 MFMA products."""
     p = _lib.KParams()
     p.width, p.height, p.stride = width, height, stride or width
     p.block_step, p.bm_range = block_step, bm_range
@@ -49,6 +50,7 @@ def kparams(width, height, stride=None, *, block_step=8, bm_range=9, radius=0, p
     p.t2d = TRANSFORMS[t2d.lower()] if isinstance(t2d, str) else int(t2d)
     p.t1d = TRANSFORMS[t1d.lower()] if isinstance(t1d, str) else int(t1d)
     p.proc_mask = proc_mask
+    p.transform_precision = {"fp32": 0, "split-bf16": 1}[precision]
     return p
 
 
