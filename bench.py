@@ -116,6 +116,76 @@ def profile_target(args):
     print(f"bench: {args.steps} matching launches", flush=True)
 
 
+def bench_batch(args):
+    """64 independent Gray planes of seeded uneven sizes and sigmas, two-pass BM3D (bs 4, R 12,
+    extractor 3) as one batched plan: one launch per stage, replayed as one hipGraph, filter
+    tiles from the autotuner. Compared with the same planes run one after another."""
+    import torch
+    from vszipcu import device, synth
+    rng = np.random.default_rng(4)
+    sizes = [(int(rng.choice([320, 640, 1280, 1920, 3840])), int(rng.choice([240, 360, 720, 1080, 2160])))
+             for _ in range(64)]
+    sigmas = [float(rng.uniform(1.0, 20.0)) for _ in range(64)]
+    print(f"bench: batch of 64 planes, {sum(w * h for w, h in sizes) / 1e6:.1f} Mpix", file=sys.stderr, flush=True)
+    clean = {}
+    for (w, h) in sorted(set(sizes)):
+        clean[(w, h)] = torch.from_numpy(synth.clean_luma(w, h, w * 7 + h)).cuda()
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    noisy = [clean[sz] + torch.randn(sz[1], sz[0], generator=gen, device="cuda") * (sg / 255.0)
+             for sz, sg in zip(sizes, sigmas)]
+    outs = [torch.zeros_like(t) for t in noisy]
+    planes = [(w, h, w, sg) for (w, h), sg in zip(sizes, sigmas)]
+    t0 = time.perf_counter()
+    b = device.Batch(planes, block_step=4, bm_range=12, extractor_exp=3, autotune=True)
+    work = torch.zeros(b.workspace_size, dtype=torch.uint8, device="cuda")
+    b.run(noisy, outs, work)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        b.run(noisy, outs, work)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run(noisy, outs, work)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    px = sum(w * h for w, h in sizes)
+    # the same planes one after another through the single-plane entry point
+    kw = dict(block_step=4, bm_range=12, extractor_exp=3)
+    seq = []
+    for (w, h), sg in zip(sizes, sigmas):
+        kb = device.kparams(w, h, sigma=device.scaled_sigma([sg] * 3, False), **kw)
+        kf = device.kparams(w, h, sigma=device.scaled_sigma([sg] * 3, True), final=True, **kw)
+        seq.append((kb, kf))
+    wmax = max(device.workspace_size(kf) for _, kf in seq)
+    w1 = torch.zeros(wmax, dtype=torch.uint8, device="cuda")
+    basic = torch.zeros(max(w * h for w, h in sizes), dtype=torch.float32, device="cuda")
+
+    def sequential():
+        for (kb, kf), t, o in zip(seq, noisy, outs):
+            bv = basic[: kb.height * kb.width].view(1, kb.height, kb.width)
+            device.run(kb, t[None, None], None, w1, bv, 0)
+            device.run(kf, bv[:, None], t[None, None], w1, o[None], 0)
+
+    sequential()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        sequential()
+    torch.cuda.synchronize()
+    dt_seq = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    tiles = sorted({b.tile(i, 0) for i in range(64)} | {b.tile(i, 1) for i in range(64)})
+    print(json.dumps({"metric": "BM3D batched uneven planes Mpix/s", "value": round(px / dt / 1e6, 3),
+                      "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "dtype": "f32",
+                      "data": "synthetic",
+                      "config": {"workload": "64 Gray planes, sizes {320..3840}x{240..2160} (seed 4), sigma U[1,20], "
+                                             "two-pass, bs 4, R 12, extractor 3, one hipGraph per run",
+                                 "mpix": round(px / 1e6, 2), "sequential_ms": round(dt_seq * 1e3, 3),
+                                 "sequential_mpix_s": round(px / dt_seq / 1e6, 3),
+                                 "setup_s": round(t_setup, 2), "autotuned_tiles": tiles}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +194,8 @@ def main():
     ap.add_argument("--cpu-strip-rows", type=int, default=136, help="rows of the single-thread CPU sample")
     ap.add_argument("--no-end-to-end", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["frame1080p", "batch64"], default="frame1080p",
+                    help="batch64: 64 uneven Gray planes in one batched plan (not the headline line)")
     ap.add_argument("--profile-target", choices=["step", "match", "copy"], default="step",
                     help="match: only the basic-pass matching of the bench frame, --steps launches (for "
                          "rocprofv3 --pmc runs); copy: a 256 MiB device copy, 3 times (counter calibration)")
@@ -139,6 +211,8 @@ def main():
     assert torch.cuda.is_available(), "bench needs a GPU"
     if args.profile_target != "step":
         return profile_target(args)
+    if args.workload == "batch64":
+        return bench_batch(args)
     print(f"bench: workload 1920x1080 YUV444 two-pass BM3D, steps={args.steps} warmup={args.warmup}", file=sys.stderr,
           flush=True)
     clean, nz = synth.yuv444_frame(W, H, SIGMA, seed=2)

@@ -92,6 +92,38 @@ int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d
 int vsz_vaggregate_run(int width, int height, int stride, int radius, int n, int num_frames,
                        const float *const *d_slots, float *d_dst, void *stream);
 
+/* ---- Batched independent planes -------------------------------------------------------------
+ * Many Gray planes of different sizes and sigmas denoised together: every stage (clear, match,
+ * filter, finish) is ONE launch over all planes, and a run is one hipGraph (captured on the
+ * first run, replayed afterwards). No reference counterpart as one call: it batches the
+ * per-entry launch loop of src/bm3d.zig:375-401 across planes. */
+typedef struct vsz_bm3d_plane {
+    int width, height, stride;   /* elements */
+    float sigma;                 /* the filter's raw `sigma` (scaled internally per pass) */
+} vsz_bm3d_plane;
+
+typedef struct vsz_bm3d_batch_params {
+    int block_step, bm_range, extractor_exp;
+    int metric, t2d, t1d;        /* t2d = t1d = 0 (dct) */
+    int transform_precision;     /* as in vsz_bm3d_kparams */
+    int two_pass;                /* 1: basic, then Wiener with ref = basic; 0: basic only */
+    int autotune;                /* 1: time every fitting filter tile shape per plane shape on the
+                                    first run and keep the fastest; 0: cost model */
+} vsz_bm3d_batch_params;
+
+typedef struct vsz_bm3d_batch vsz_bm3d_batch;
+
+int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane *planes, int n,
+                          vsz_bm3d_batch **out);
+size_t vsz_bm3d_batch_workspace_size(const vsz_bm3d_batch *b);
+/* d_noisy[i]: [height][stride] input, d_out[i]: [height][stride] result of plane i. The first
+ * run tunes (when asked) and captures the graph; a run with other pointers re-captures. */
+int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *const *d_out, void *d_work,
+                       void *stream);
+/* Filter tile (in reference blocks) chosen for plane i, pass 0 (basic) or 1 (final). */
+int vsz_bm3d_batch_tile(const vsz_bm3d_batch *b, int plane, int pass, int *tbx, int *tby);
+void vsz_bm3d_batch_free(vsz_bm3d_batch *b);
+
 /* =============================================================================================
  * Filter layer
  * ============================================================================================= */

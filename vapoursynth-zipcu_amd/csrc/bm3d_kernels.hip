@@ -20,7 +20,13 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
 #include "vszipcu_bm3d.h"
 #include "internal.h"
@@ -425,10 +431,8 @@ __device__ __forceinline__ void lds_step(float (&w)[9][8], const char *lds, cons
 }
 
 template <int M>
-__global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
-    extern __shared__ float region[];
+__device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, float *region) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int W = a.width, H = a.height, S = a.stride, bs = a.block_step, Rg = a.bm_range;
     const int bx0 = tx * 16, by0 = ty * 16;
@@ -519,6 +523,12 @@ __global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
     int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
+    extern __shared__ float region[];
+    match_lds_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), region);
 }
 
 template <int M, bool TEMPORAL>
@@ -875,15 +885,14 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
 // is flushed once with 64-bit device atomics; integer addition commutes, so the result does
 // not depend on the order of any of these updates.
 template <bool FINAL, int T2, int T1, bool EXACT>
-__global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
-    extern __shared__ float smem[];
+__device__ __forceinline__ void filter_tile_body(const FilterArgs &a, int tile_x, int tile_y, float *smem) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 3, c = lane & 7;
     float *buf = smem + (wave * 8 + g) * 72;
     int *cell_i = reinterpret_cast<int *>(smem + 4 * 8 * 72);
     float *cell_f = smem + 4 * 8 * 72;
 
     const int W = a.width, H = a.height, bs = a.block_step, R = a.bm_range;
-    const int bx0 = blockIdx.x * a.tbx, by0 = blockIdx.y * a.tby;
+    const int bx0 = tile_x * a.tbx, by0 = tile_y * a.tby;
     const int nbx_t = min(a.tbx, a.nbx - bx0), nby_t = min(a.tby, a.nby - by0);
     const int ngroups = nbx_t * nby_t;
     const int rx0 = max(min(bx0 * bs, W - 8) - R, 0);
@@ -966,16 +975,22 @@ __global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
     }
 }
 
+template <bool FINAL, int T2, int T1, bool EXACT>
+__global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
+    extern __shared__ float smem[];
+    filter_tile_body<FINAL, T2, T1, EXACT>(a, blockIdx.x, blockIdx.y, smem);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Finishing: numerator as accumulated, denominator as the exact 8x8 box sum of the corner
 // scatter, each rounded once to FP32; then either the IEEE division (src/bm3d.cu:898-922) or
 // the stacked [P][TW][2][H][S] payload. One workgroup per 64x16 output tile.
 // ------------------------------------------------------------------------------------------------
 template <bool EXACT>
-__global__ __launch_bounds__(256) void k_finish(FinishArgs a) {
+__device__ __forceinline__ void finish_body(const FinishArgs &a, int tile_x, int tile_y, int pz,
+                                            typename std::conditional<EXACT, long long, float>::type (*sD)[72]) {
     using T = typename std::conditional<EXACT, long long, float>::type;
-    __shared__ T sD[23][72];
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 16, pz = blockIdx.z;
+    const int x0 = tile_x * 64, y0 = tile_y * 16;
     const int p = pz / a.tw;
     if (a.out_kind == 0 && !((a.proc_mask >> p) & 1)) return;
     const long long base = (long long)pz * 2 * a.ts;
@@ -1015,6 +1030,12 @@ __global__ __launch_bounds__(256) void k_finish(FinishArgs a) {
     }
 }
 
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_finish(FinishArgs a) {
+    __shared__ typename std::conditional<EXACT, long long, float>::type sD[23][72];
+    finish_body<EXACT>(a, blockIdx.x, blockIdx.y, blockIdx.z, sD);
+}
+
 __global__ __launch_bounds__(256) void k_zero(unsigned long long *p, long long n64) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n64; i += (long long)gridDim.x * 256) p[i] = 0ull;
 }
@@ -1032,6 +1053,65 @@ __global__ __launch_bounds__(256) void k_vaggregate(VaggArgs a) {
         sw = sw + base[a.ts + i];
     }
     a.dst[i] = __fdiv_rn(sv, sw);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batched planes: one launch per stage covers every plane of a batch. Workgroup g belongs to
+// the plane p with off[p] <= g < off[p+1]; its arguments come from a device-side table.
+// ------------------------------------------------------------------------------------------------
+constexpr int kMaxBatch = 256;
+
+struct BatchTable {
+    int n;
+    int match_off[kMaxBatch + 1];
+    int filter_off[kMaxBatch + 1];
+    int finish_off[kMaxBatch + 1];
+    int filter_tiles_x[kMaxBatch];
+    int finish_tiles_x[kMaxBatch];
+    int finish_tiles_y[kMaxBatch];
+    MatchArgs m[kMaxBatch];
+    FilterArgs f[kMaxBatch];
+    FinishArgs fin[kMaxBatch];
+};
+
+// Largest p with off[p] <= g (off ascending, off[0] = 0).
+__device__ __forceinline__ int batch_plane(const int *off, int n, int g) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_match_lds_batch(const BatchTable *__restrict__ t) {
+    extern __shared__ float region[];
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    const int p = batch_plane(t->match_off, t->n, g);
+    const MatchArgs &a = t->m[p];
+    match_lds_body<M>(a, g - t->match_off[p], region);
+}
+
+template <bool FINAL, int T2, int T1, bool EXACT>
+__global__ __launch_bounds__(256, 2) void k_filter_tile_batch(const BatchTable *__restrict__ t) {
+    extern __shared__ float smem[];
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    const int p = batch_plane(t->filter_off, t->n, g);
+    const FilterArgs &a = t->f[p];
+    const int local = g - t->filter_off[p], tx = t->filter_tiles_x[p];
+    filter_tile_body<FINAL, T2, T1, EXACT>(a, local % tx, local / tx, smem);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_finish_batch(const BatchTable *__restrict__ t) {
+    __shared__ typename std::conditional<EXACT, long long, float>::type sD[23][72];
+    const int g = blockIdx.x;
+    const int p = batch_plane(t->finish_off, t->n, g);
+    const FinishArgs &a = t->fin[p];
+    const int local = g - t->finish_off[p], tx = t->finish_tiles_x[p], ty = t->finish_tiles_y[p];
+    finish_body<EXACT>(a, local % tx, (local / tx) % ty, local / (tx * ty), sD);
 }
 
 }  // namespace vszk
@@ -1128,10 +1208,18 @@ struct TilePlan {
     size_t lds_bytes = 0;
 };
 
+// Exact int32 LDS cells take at most 8*tbx*tby contributions per plane; each must stay below
+// 2^guard_bits so that the cell cannot overflow: (8*tbx*tby) * 2^guard_bits <= 2^31.
+int guard_bits_for(int tbx, int tby) {
+    int lg = 0;
+    while ((1LL << lg) < 8LL * tbx * tby) ++lg;
+    return 31 - lg;
+}
+
 // Picks the block tile of the LDS-accumulating filter kernel: the region a tile's groups can
 // reach must fit the LDS budget; among those, the tiling with the least flush traffic (tiles x
 // region cells) that still yields enough workgroups to fill the chip.
-TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L) {
+TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L, int min_wgs = 1024) {
     TilePlan best;
     if (p->radius != 0) return best;
     const int opts[] = {32, 16, 8, 4, 2, 1};
@@ -1147,7 +1235,7 @@ TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L) {
             if (bytes > kRegionBudgetBytes) continue;
             const long long wgs = (long long)((L.nbx + tbx - 1) / tbx) * ((L.nby + tby - 1) / tby);
             long long cost = wgs * rw * rh;
-            if (wgs < 1024) cost += (1024 - wgs) * rw * rh * 4;  // under-filled chip
+            if (wgs < min_wgs) cost += (min_wgs - wgs) * rw * rh * 4;  // under-filled chip
             if (best_cost < 0 || cost < best_cost) {
                 best_cost = cost;
                 best.ok = true;
@@ -1156,12 +1244,32 @@ TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L) {
                 best.lds_bytes = bytes;
             }
         }
-    if (best.ok) {
-        int lg = 0;
-        while ((1LL << lg) < 8LL * best.tbx * best.tby) ++lg;
-        best.guard_bits = 31 - lg;  // (8*tbx*tby) * 2^guard_bits <= 2^31
-    }
+    if (best.ok) best.guard_bits = guard_bits_for(best.tbx, best.tby);
     return best;
+}
+
+// Every candidate tile shape that fits the LDS budget (the autotuner's search space).
+std::vector<TilePlan> tile_candidates(const vsz_bm3d_kparams *p, const Layout &L) {
+    std::vector<TilePlan> out;
+    if (p->radius != 0) return out;
+    const int opts[] = {32, 16, 8, 4, 2, 1};
+    for (int tbx : opts)
+        for (int tby : opts) {
+            if ((tbx > L.nbx * 2 && tbx > 1) || (tby > L.nby * 2 && tby > 1) || tbx * tby > 512 || tbx * tby < 16)
+                continue;
+            const long long rw = std::min<long long>((long long)(tbx - 1) * p->block_step + 8 + 2LL * p->bm_range, p->width);
+            const long long rh = std::min<long long>((long long)(tby - 1) * p->block_step + 8 + 2LL * p->bm_range, p->height);
+            const size_t bytes = kTransposeBytes + (size_t)(rw * rh) * 4;
+            if (bytes > kRegionBudgetBytes) continue;
+            TilePlan t;
+            t.ok = true;
+            t.tbx = tbx;
+            t.tby = tby;
+            t.lds_bytes = bytes;
+            t.guard_bits = guard_bits_for(tbx, tby);
+            out.push_back(t);
+        }
+    return out;
 }
 
 template <bool FINAL, int T2, int T1, bool EXACT>
@@ -1343,5 +1451,420 @@ int vsz_vaggregate_run(int width, int height, int stride, int radius, int n, int
     hipLaunchKernelGGL(k_vaggregate, grid, dim3(256), 0, (hipStream_t)stream, a);
     return check_launch("vsz_vaggregate_run");
 }
+
+}  // extern "C"
+
+// ================================================================================================
+// Batched planes: plan, autotuner, graph
+// ================================================================================================
+namespace {
+
+struct BatchPlane {
+    vsz_bm3d_kparams k[2]{};  // basic, final
+    Layout L{};
+    TilePlan tile[2];
+    size_t acc_off = 0, match_off = 0, basic_off = 0;
+};
+
+float bits_float(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// Autotuner results, shared by every plan of the process: (shape, pass, kernel variant) -> tile.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int, int, int, int>, std::pair<int, int>> g_tune;
+
+template <bool FINAL, int T, bool EXACT>
+void launch_filter_batch(const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_filter_tile_batch<FINAL, T, T, EXACT>), dim3(wgs), dim3(256), lds, s, d);
+}
+
+void launch_match_batch(int metric, const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
+    switch (metric) {
+    case 0: hipLaunchKernelGGL((k_match_lds_batch<0>), dim3(wgs), dim3(256), lds, s, d); break;
+    case 1: hipLaunchKernelGGL((k_match_lds_batch<1>), dim3(wgs), dim3(256), lds, s, d); break;
+    case 2: hipLaunchKernelGGL((k_match_lds_batch<2>), dim3(wgs), dim3(256), lds, s, d); break;
+    case 3: hipLaunchKernelGGL((k_match_lds_batch<3>), dim3(wgs), dim3(256), lds, s, d); break;
+    default: hipLaunchKernelGGL((k_match_lds_batch<4>), dim3(wgs), dim3(256), lds, s, d); break;
+    }
+}
+
+}  // namespace
+
+struct vsz_bm3d_batch {
+    vsz_bm3d_batch_params bp{};
+    std::vector<BatchPlane> planes;
+    bool exact = false;
+    size_t acc_total = 0, work_bytes = 0;
+    size_t match_lds = 0;
+    int match_stride = 0;
+    bool tuned = false;
+    BatchTable *h_tab[2] = {nullptr, nullptr};
+    BatchTable *d_tab = nullptr;  // [2]
+    const void *last_in = nullptr, *last_out = nullptr, *last_work = nullptr;
+    std::vector<const float *> last_noisy;
+    std::vector<float *> last_dst;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~vsz_bm3d_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        if (d_tab) (void)hipFree(d_tab);
+        delete h_tab[0];
+        delete h_tab[1];
+    }
+
+    int passes() const { return bp.two_pass ? 2 : 1; }
+    int filter_wgs(int pass) const {
+        int n = 0;
+        for (const auto &pl : planes)
+            n += ((pl.L.nbx + pl.tile[pass].tbx - 1) / pl.tile[pass].tbx) * ((pl.L.nby + pl.tile[pass].tby - 1) / pl.tile[pass].tby);
+        return n;
+    }
+    size_t filter_lds(int pass) const {
+        size_t m = 0;
+        for (const auto &pl : planes) m = std::max(m, pl.tile[pass].lds_bytes);
+        return m;
+    }
+    int match_wgs() const {
+        int n = 0;
+        for (const auto &pl : planes) n += ((pl.L.nbx + 15) / 16) * ((pl.L.nby + 15) / 16);
+        return n;
+    }
+    int finish_wgs() const {
+        int n = 0;
+        for (const auto &pl : planes) n += ((pl.k[0].width + 63) / 64) * ((pl.k[0].height + 15) / 16);
+        return n;
+    }
+
+    void fill_tables(const float *const *noisy, float *const *dst, char *work) {
+        for (int pass = 0; pass < passes(); ++pass) {
+            BatchTable &t = *h_tab[pass];
+            t.n = (int)planes.size();
+            int mo = 0, fo = 0, so = 0;
+            for (size_t i = 0; i < planes.size(); ++i) {
+                const BatchPlane &pl = planes[i];
+                const vsz_bm3d_kparams &k = pl.k[pass];
+                const bool last = pass == passes() - 1;
+                const float *clip0 = pass == 0 ? noisy[i] : reinterpret_cast<const float *>(work + pl.basic_off);
+                float *out = last ? dst[i] : reinterpret_cast<float *>(work + pl.basic_off);
+                int4 *matches = reinterpret_cast<int4 *>(work + pl.match_off);
+                void *acc = work + pl.acc_off;
+                t.match_off[i] = mo;
+                t.filter_off[i] = fo;
+                t.finish_off[i] = so;
+                MatchArgs &m = t.m[i];
+                m = MatchArgs{};
+                m.img = clip0;
+                m.matches = matches;
+                m.width = k.width;
+                m.height = k.height;
+                m.stride = k.stride;
+                m.nbx = pl.L.nbx;
+                m.nby = pl.L.nby;
+                m.block_step = k.block_step;
+                m.bm_range = k.bm_range;
+                m.ps_num = k.ps_num;
+                m.ps_range = k.ps_range;
+                m.ts = pl.L.ts;
+                m.tiles_x = (pl.L.nbx + 15) / 16;
+                m.lds_stride = match_stride;
+                mo += m.tiles_x * ((pl.L.nby + 15) / 16);
+                FilterArgs &f = t.f[i];
+                f = FilterArgs{};
+                f.matches = matches;
+                f.clip0 = clip0;
+                f.clip1 = noisy[i];
+                f.acc = acc;
+                f.nblocks = pl.L.nblocks;
+                f.num_planes = 1;
+                f.stride = k.stride;
+                f.ts = pl.L.ts;
+                f.plane_elems = pl.L.ts;
+                f.acc_plane_elems = 2 * pl.L.ts;
+                f.sigma[0] = f.sigma[1] = f.sigma[2] = k.sigma[0];
+                f.active_mask = 1;
+                f.exact = exact ? 1 : 0;
+                f.extractor_exp = k.extractor_exp;
+                f.extractor = k.extractor_exp != 0 ? ldexpf(1.0f, k.extractor_exp) : 0.0f;
+                f.width = k.width;
+                f.height = k.height;
+                f.block_step = k.block_step;
+                f.bm_range = k.bm_range;
+                f.nbx = pl.L.nbx;
+                f.nby = pl.L.nby;
+                f.tbx = pl.tile[pass].tbx;
+                f.tby = pl.tile[pass].tby;
+                f.guard_bits = pl.tile[pass].guard_bits;
+                t.filter_tiles_x[i] = (pl.L.nbx + f.tbx - 1) / f.tbx;
+                fo += t.filter_tiles_x[i] * ((pl.L.nby + f.tby - 1) / f.tby);
+                FinishArgs &fa = t.fin[i];
+                fa = FinishArgs{};
+                fa.acc = acc;
+                fa.out = out;
+                fa.width = k.width;
+                fa.height = k.height;
+                fa.stride = k.stride;
+                fa.tw = 1;
+                fa.out_kind = 0;
+                fa.ts = pl.L.ts;
+                fa.proc_mask = 1;
+                fa.extractor_exp = k.extractor_exp;
+                t.finish_tiles_x[i] = (k.width + 63) / 64;
+                t.finish_tiles_y[i] = (k.height + 15) / 16;
+                so += t.finish_tiles_x[i] * t.finish_tiles_y[i];
+            }
+            t.match_off[planes.size()] = mo;
+            t.filter_off[planes.size()] = fo;
+            t.finish_off[planes.size()] = so;
+        }
+    }
+
+    // All launches of one run, on stream s (captured into the graph).
+    void enqueue(char *work, hipStream_t s) {
+        const long long n64 = (long long)(acc_total / 8);
+        const unsigned zgrid = (unsigned)std::min<long long>((n64 + 255) / 256, 8192);
+        for (int pass = 0; pass < passes(); ++pass) {
+            const BatchTable *d = d_tab + pass;
+            hipLaunchKernelGGL(k_zero, dim3(zgrid), dim3(256), 0, s, (unsigned long long *)work, n64);
+            launch_match_batch(bp.metric, d, match_wgs(), match_lds, s);
+            const int fw = filter_wgs(pass);
+            const size_t fl = filter_lds(pass);
+            const bool mf = bp.transform_precision == 1;
+            if (pass == 0) {
+                if (mf) exact ? launch_filter_batch<false, kDctMfma, true>(d, fw, fl, s) : launch_filter_batch<false, kDctMfma, false>(d, fw, fl, s);
+                else exact ? launch_filter_batch<false, 0, true>(d, fw, fl, s) : launch_filter_batch<false, 0, false>(d, fw, fl, s);
+            } else {
+                if (mf) exact ? launch_filter_batch<true, kDctMfma, true>(d, fw, fl, s) : launch_filter_batch<true, kDctMfma, false>(d, fw, fl, s);
+                else exact ? launch_filter_batch<true, 0, true>(d, fw, fl, s) : launch_filter_batch<true, 0, false>(d, fw, fl, s);
+            }
+            if (exact) hipLaunchKernelGGL(k_finish_batch<true>, dim3(finish_wgs()), dim3(256), 0, s, d);
+            else hipLaunchKernelGGL(k_finish_batch<false>, dim3(finish_wgs()), dim3(256), 0, s, d);
+        }
+    }
+
+    int tune(const float *const *noisy, char *work, hipStream_t s);
+};
+
+// Times every candidate tile shape of the filter kernel on each distinct plane shape (both
+// passes, the plane's own noisy data as input, its own workspace) and keeps the fastest.
+int vsz_bm3d_batch::tune(const float *const *noisy, char *work, hipStream_t s) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipEventCreate failed");
+    int rc = VSZ_OK;
+    for (size_t i = 0; i < planes.size() && rc == VSZ_OK; ++i) {
+        BatchPlane &pl = planes[i];
+        for (int pass = 0; pass < passes() && rc == VSZ_OK; ++pass) {
+            const vsz_bm3d_kparams &k = pl.k[pass];
+            const auto key = std::make_tuple(k.width, k.height, k.block_step, k.bm_range, pass, bp.metric,
+                                             bp.transform_precision, exact ? 1 : 0);
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tune.find(key);
+                if (it != g_tune.end()) {
+                    for (const TilePlan &c : tile_candidates(&k, pl.L))
+                        if (c.tbx == it->second.first && c.tby == it->second.second) pl.tile[pass] = c;
+                    continue;
+                }
+            }
+            // matches for this plane (the final pass is timed on the noisy plane as its ref)
+            int4 *matches = reinterpret_cast<int4 *>(work + pl.match_off);
+            rc = vsz_bm3d_match(&k, noisy[i], reinterpret_cast<int32_t *>(matches), s);
+            if (rc) break;
+            FilterArgs f{};
+            f.matches = matches;
+            f.clip0 = noisy[i];
+            f.clip1 = noisy[i];
+            f.acc = work + pl.acc_off;
+            f.nblocks = pl.L.nblocks;
+            f.num_planes = 1;
+            f.stride = k.stride;
+            f.ts = pl.L.ts;
+            f.plane_elems = pl.L.ts;
+            f.acc_plane_elems = 2 * pl.L.ts;
+            f.sigma[0] = f.sigma[1] = f.sigma[2] = k.sigma[0];
+            f.active_mask = 1;
+            f.exact = exact ? 1 : 0;
+            f.extractor_exp = k.extractor_exp;
+            f.extractor = k.extractor_exp != 0 ? ldexpf(1.0f, k.extractor_exp) : 0.0f;
+            f.width = k.width;
+            f.height = k.height;
+            f.block_step = k.block_step;
+            f.bm_range = k.bm_range;
+            f.nbx = pl.L.nbx;
+            f.nby = pl.L.nby;
+            float best_ms = -1.0f;
+            TilePlan best = pl.tile[pass];
+            for (const TilePlan &c : tile_candidates(&k, pl.L)) {
+                f.tbx = c.tbx;
+                f.tby = c.tby;
+                f.guard_bits = c.guard_bits;
+                const int T = bp.transform_precision == 1 ? kDctMfma : 0;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 3 && rc == VSZ_OK; ++rep) {
+                    (void)hipEventRecord(e0, s);
+                    if (T == kDctMfma) {
+                        if (pass == 0) launch_filter_3<false, kDctMfma, kDctMfma>(f, c, s);
+                        else launch_filter_3<true, kDctMfma, kDctMfma>(f, c, s);
+                    } else {
+                        if (pass == 0) launch_filter_3<false, 0, 0>(f, c, s);
+                        else launch_filter_3<true, 0, 0>(f, c, s);
+                    }
+                    (void)hipEventRecord(e1, s);
+                    if (hipEventSynchronize(e1) != hipSuccess) rc = vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: tuning launch failed");
+                    float t = 0.0f;
+                    (void)hipEventElapsedTime(&t, e0, e1);
+                    if (rep > 0) ms += t;  // the first launch warms caches
+                }
+                if (rc) break;
+                if (best_ms < 0.0f || ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+            pl.tile[pass] = best;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[key] = std::make_pair(best.tbx, best.tby);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+extern "C" {
+
+int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane *planes, int n, vsz_bm3d_batch **out) {
+    if (!bp || !planes || !out || n < 1 || n > kMaxBatch)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: need 1..%d planes", kMaxBatch);
+    if (bp->t2d != 0 || bp->t1d != 0 || bp->transform_precision < 0 || bp->transform_precision > 1)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: batches use the dct/dct transforms");
+    auto b = std::make_unique<vsz_bm3d_batch>();
+    b->bp = *bp;
+    b->exact = vsz_exact_extractor(bp->extractor_exp);
+    int ls = 0;
+    b->match_lds = match_lds_bytes(bp->block_step, bp->bm_range, &ls);
+    b->match_stride = ls;
+    if (!b->match_lds)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: block_step/bm_range too large for the LDS-staged matcher");
+    const float fac[2] = {bits_float(0x3f021bb6u), bits_float(0x3e40c0c1u)};  // src/bm3d.zig:1013-1024
+    size_t acc = 0, rest = 0;
+    for (int i = 0; i < n; ++i) {
+        BatchPlane pl;
+        if (!(planes[i].sigma >= FLT_EPSILON))
+            return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: sigma must be >= FLT_EPSILON", i);
+        for (int pass = 0; pass < 2; ++pass) {
+            vsz_bm3d_kparams &k = pl.k[pass];
+            k.width = planes[i].width;
+            k.height = planes[i].height;
+            k.stride = planes[i].stride;
+            k.block_step = bp->block_step;
+            k.bm_range = bp->bm_range;
+            k.radius = 0;
+            k.ps_num = 2;
+            k.ps_range = 4;
+            k.final_pass = pass;
+            k.sigma[0] = k.sigma[1] = k.sigma[2] = planes[i].sigma * fac[pass];
+            k.extractor_exp = bp->extractor_exp;
+            k.metric = bp->metric;
+            k.proc_mask = 1;
+            k.transform_precision = bp->transform_precision;
+            if (!kparams_ok(&k)) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: invalid geometry", i);
+            pl.L = layout_of(&k);
+            pl.tile[pass] = plan_tiles(&k, pl.L, 0);
+            if (!pl.tile[pass].ok) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: no filter tile fits", i);
+        }
+        pl.acc_off = acc;
+        acc += (pl.L.acc_bytes + 255) & ~size_t(255);
+        b->planes.push_back(pl);
+    }
+    b->acc_total = acc;
+    rest = acc;
+    for (auto &pl : b->planes) {
+        pl.match_off = rest;
+        rest += pl.L.match_bytes;
+        pl.basic_off = rest;
+        rest += ((size_t)pl.L.ts * 4 + 255) & ~size_t(255);
+    }
+    b->work_bytes = rest;
+    b->h_tab[0] = new BatchTable();
+    b->h_tab[1] = new BatchTable();
+    if (hipMalloc(&b->d_tab, 2 * sizeof(BatchTable)) != hipSuccess) {
+        b->d_tab = nullptr;
+        return vsz_fail(VSZ_ENOMEM, "vsz_bm3d_batch_create: hipMalloc failed");
+    }
+    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
+        b->cap = nullptr;
+        return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_create: hipStreamCreate failed");
+    }
+    b->tuned = !bp->autotune;
+    *out = b.release();
+    return VSZ_OK;
+}
+
+size_t vsz_bm3d_batch_workspace_size(const vsz_bm3d_batch *b) { return b ? b->work_bytes : 0; }
+
+int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *const *d_out, void *d_work, void *stream) {
+    if (!b || !d_noisy || !d_out || !d_work) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_run: invalid arguments");
+    const size_t n = b->planes.size();
+    for (size_t i = 0; i < n; ++i)
+        if (!d_noisy[i] || !d_out[i]) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_run: null plane %zu", i);
+    hipStream_t s = (hipStream_t)stream;
+    char *work = static_cast<char *>(d_work);
+    if (!b->tuned) {
+        const int rc = b->tune(d_noisy, work, s);
+        if (rc) return rc;
+        b->tuned = true;
+    }
+    const bool same = b->last_work == d_work && b->last_noisy.size() == n &&
+                      std::equal(b->last_noisy.begin(), b->last_noisy.end(), d_noisy) &&
+                      std::equal(b->last_dst.begin(), b->last_dst.end(), d_out);
+    if (!same) {
+        b->fill_tables(d_noisy, d_out, work);
+        for (int pass = 0; pass < b->passes(); ++pass)
+            if (hipMemcpyAsync(b->d_tab + pass, b->h_tab[pass], sizeof(BatchTable), hipMemcpyHostToDevice, s) != hipSuccess)
+                return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
+        // the pageable source may be reused as soon as the copy is done
+        if (hipStreamSynchronize(s) != hipSuccess) return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
+        b->last_work = d_work;
+        b->last_noisy.assign(d_noisy, d_noisy + n);
+        b->last_dst.assign(d_out, d_out + n);
+        if (b->exec) {  // the captured launches read the workspace base through the zero kernel
+            (void)hipGraphExecDestroy(b->exec);
+            (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+        }
+    }
+    if (!b->exec) {
+        if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipStreamBeginCapture failed");
+        b->enqueue(work, b->cap);
+        if (hipStreamEndCapture(b->cap, &b->graph) != hipSuccess || !b->graph)
+            return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: graph capture failed");
+        if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+            b->exec = nullptr;
+            return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipGraphInstantiate failed");
+        }
+    }
+    if (hipGraphLaunch(b->exec, s) != hipSuccess) return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipGraphLaunch failed");
+    return VSZ_OK;
+}
+
+int vsz_bm3d_batch_tile(const vsz_bm3d_batch *b, int plane, int pass, int *tbx, int *tby) {
+    if (!b || plane < 0 || plane >= (int)b->planes.size() || pass < 0 || pass > 1 || !tbx || !tby)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_tile: invalid arguments");
+    *tbx = b->planes[plane].tile[pass].tbx;
+    *tby = b->planes[plane].tile[pass].tby;
+    return VSZ_OK;
+}
+
+void vsz_bm3d_batch_free(vsz_bm3d_batch *b) { delete b; }
 
 }  // extern "C"

@@ -69,6 +69,20 @@ class OutputInfo(C.Structure):
     ]
 
 
+class Plane(C.Structure):
+    """vsz_bm3d_plane"""
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("stride", C.c_int), ("sigma", C.c_float)]
+
+
+class BatchParams(C.Structure):
+    """vsz_bm3d_batch_params"""
+    _fields_ = [
+        ("block_step", C.c_int), ("bm_range", C.c_int), ("extractor_exp", C.c_int),
+        ("metric", C.c_int), ("t2d", C.c_int), ("t1d", C.c_int), ("transform_precision", C.c_int),
+        ("two_pass", C.c_int), ("autotune", C.c_int),
+    ]
+
+
 GET_PLANE_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ssize_t))
 
 # Every function include/vszipcu_bm3d.h declares: name -> (restype, argtypes)
@@ -90,6 +104,12 @@ PROTOTYPES = {
     "vsz_filter_output_info": (C.c_int, [C.c_void_p, C.POINTER(OutputInfo)]),
     "vsz_filter_get_frame": (C.c_int, [C.c_void_p, C.c_int, GET_PLANE_FN, C.c_void_p, C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_ssize_t), C.c_char_p, C.c_size_t]),
+    "vsz_bm3d_batch_create": (C.c_int, [C.POINTER(BatchParams), C.POINTER(Plane), C.c_int, C.POINTER(C.c_void_p)]),
+    "vsz_bm3d_batch_workspace_size": (C.c_size_t, [C.c_void_p]),
+    "vsz_bm3d_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                     C.c_void_p]),
+    "vsz_bm3d_batch_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vsz_bm3d_batch_free": (None, [C.c_void_p]),
     "vsz_filter_frame_range": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vsz_filter_free": (None, [C.c_void_p]),
 }

@@ -128,5 +128,57 @@ def vaggregate(width: int, radius: int, n: int, num_frames: int, slots: Sequence
                                           _stream_handle(stream)))
 
 
+class Batch:
+    """Many independent Gray planes, two-pass BM3D, one launch per stage, replayed as one hipGraph.
+
+    planes: [(width, height, stride, sigma)] with the filter's raw sigma per plane."""
+
+    def __init__(self, planes, *, block_step=4, bm_range=12, extractor_exp=3, metric="ssd", precision="fp32",
+                 two_pass=True, autotune=True):
+        bp = _lib.BatchParams()
+        bp.block_step, bp.bm_range, bp.extractor_exp = block_step, bm_range, extractor_exp
+        bp.metric = METRICS[metric.lower()] if isinstance(metric, str) else int(metric)
+        bp.t2d = bp.t1d = 0
+        bp.transform_precision = {"fp32": 0, "split-bf16": 1}[precision]
+        bp.two_pass, bp.autotune = int(bool(two_pass)), int(bool(autotune))
+        arr = (_lib.Plane * len(planes))()
+        for i, (w, h, st, sg) in enumerate(planes):
+            arr[i].width, arr[i].height, arr[i].stride, arr[i].sigma = w, h, st, sg
+        self.planes = [tuple(p) for p in planes]
+        h = C.c_void_p()
+        _check(_lib.load().vsz_bm3d_batch_create(C.byref(bp), arr, len(planes), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().vsz_bm3d_batch_free(h)
+            self._h = C.c_void_p(0)
+
+    @property
+    def workspace_size(self) -> int:
+        return _lib.load().vsz_bm3d_batch_workspace_size(self._h)
+
+    def run(self, noisy: Sequence, out: Sequence, work, stream=None):
+        """noisy[i], out[i]: [height, stride] f32 device tensors; work: workspace_size bytes (u8)."""
+        n = len(self.planes)
+        if len(noisy) != n or len(out) != n:
+            raise ValueError(f"need {n} input and output planes")
+        for (w, h, st, _), a, b in zip(self.planes, noisy, out):
+            _need(a, (h, st), "f32")
+            _need(b, (h, st), "f32")
+        _need(work, None, "u8")
+        if work.numel() < self.workspace_size:
+            raise ValueError("workspace too small")
+        ins = (C.c_void_p * n)(*[t.data_ptr() for t in noisy])
+        outs = (C.c_void_p * n)(*[t.data_ptr() for t in out])
+        _check(_lib.load().vsz_bm3d_batch_run(self._h, ins, outs, C.c_void_p(work.data_ptr()), _stream_handle(stream)))
+
+    def tile(self, plane: int, pass_: int):
+        tx, ty = C.c_int(), C.c_int()
+        _check(_lib.load().vsz_bm3d_batch_tile(self._h, plane, pass_, C.byref(tx), C.byref(ty)))
+        return tx.value, ty.value
+
+
 def device_count() -> int:
     return _lib.load().vsz_device_count()
