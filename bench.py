@@ -50,6 +50,11 @@ def ssd_candidates(width, height, step, rng):
     return int(nx.sum()) * int(ny.sum())
 
 
+def _lib_cnn_ws(n, h, w):
+    from vszipcu import _lib
+    return _lib.load().vsz_cnn_workspace_size(n, h, w)
+
+
 def _oracle_two_pass(nz, rows, threads):
     import oracle
     from vszipcu import device
@@ -294,6 +299,50 @@ def main():
     print(f"bench: split-bf16 transforms {dt16 * 1e3:.3f} ms/step, {W * H / dt16 / 1e6:.2f} Mpix/s, "
           f"PSNR {psnr16:.4f} dB (plain {psnr_y:.4f})", file=sys.stderr, flush=True)
 
+    # learned + BF16 form: the CNN basic estimate (BF16 MFMA convolutions, one call for the three
+    # planes with their own noise levels), then the Wiener pass with split-BF16 DCT stages
+    learned = None
+    from vszipcu import cnn
+    if os.path.exists(cnn.DEFAULT_WEIGHTS):
+        model = cnn.Model.load()
+        sig_map = torch.tensor([s / 255.0 for s in SIGMA], dtype=torch.float32, device="cuda")
+        cwork = torch.empty(_lib_cnn_ws(3, H, W), dtype=torch.uint8, device="cuda")
+        cm = model.cmodel()
+
+        def step_learned(kpf):
+            cnn._check(cnn._lib.load().vsz_cnn_run(cnn.C.byref(cm), cnn._p(d_noisy), 3, H, W, W, cnn._p(sig_map),
+                                                   cnn._p(d_basic), cnn._p(cwork), cnn._s(stream)))
+            device.run(kpf, d_basic, d_noisy, work_f, d_final, 0, stream)
+
+        learned = {}
+        for name, kpf in (("learned_bf16", kp_f16), ("learned_fp32_wiener", kp_f)):
+            for _ in range(args.warmup):
+                step_learned(kpf)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step_learned(kpf)
+            torch.cuda.synchronize()
+            dtl = (time.perf_counter() - t0) / args.steps
+            pl = synth.psnr(d_final.cpu().numpy()[0], clean[0])
+            learned[name] = {"value": round(W * H / dtl / 1e6, 3), "ms_per_step": round(dtl * 1e3, 4),
+                             "psnr_y_db": round(pl, 4), "psnr_delta_db": round(pl - psnr_y, 4)}
+            print(f"bench: {name} {dtl * 1e3:.3f} ms/step, {W * H / dtl / 1e6:.2f} Mpix/s, PSNR {pl:.4f} dB "
+                  f"(delta {pl - psnr_y:+.4f})", file=sys.stderr, flush=True)
+        # the CNN alone, per step (3 planes)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.steps):
+            cnn._check(cnn._lib.load().vsz_cnn_run(cnn.C.byref(cm), cnn._p(d_noisy), 3, H, W, W, cnn._p(sig_map),
+                                                   cnn._p(d_basic), cnn._p(cwork), cnn._s(stream)))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_cnn = ev0.elapsed_time(ev1) / args.steps * 1e-3
+        cnn_flop = 6 * 2.0 * 9 * 32 * 32 * W * H * 3   # padded 32->32 layers, as computed
+        learned["cnn_ms"] = round(t_cnn * 1e3, 4)
+        learned["cnn_tflops"] = round(cnn_flop / t_cnn / 1e12, 2)
+        learned["weights"] = {k: model.meta.get(k) for k in ("pairs", "patch", "epochs", "steps", "train_s")}
+
     nbx, nby = device.num_blocks(kp_b)
     tf_flop = 128.0 * nbx * nby * 3 * (384 + 576)
     t_min = (flop + tf_flop) / (PEAK_FP32_TFLOPS * 1e12)
@@ -325,7 +374,7 @@ def main():
         "step_roofline": {"ssd_gflop": round(flop / 1e9, 3), "tf_gflop": round(tf_flop / 1e9, 3),
                           "t_min_ms": round(t_min * 1e3, 4), "frac": round(t_min / dt, 4)},
         "end_to_end_mpix_s": e2e,
-        "forms": {"split_bf16_dct": {"value": round(W * H / dt16 / 1e6, 3), "ms_per_step": round(dt16 * 1e3, 4),
+        "forms": {"learned": learned, "split_bf16_dct": {"value": round(W * H / dt16 / 1e6, 3), "ms_per_step": round(dt16 * 1e3, 4),
                                      "psnr_y_db": round(psnr16, 4), "psnr_delta_db": round(psnr16 - psnr_y, 4),
                                      "note": "DCT stages as compensated split-BF16 MFMA, FP32 accumulation; "
                                              "matching FP32"}},

@@ -24,12 +24,16 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_library_exports_every_declared_symbol():
     L = vszipcu.load()
-    header = open(os.path.join(REPO, "include", "vszipcu_bm3d.h")).read()
-    names = set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(vsz_\w+)\s*\(", header, re.M))
-    assert len(names) == 20
+    names = set()
+    for h in ("vszipcu_bm3d.h", "vszipcu_cnn.h"):
+        header = open(os.path.join(REPO, "include", h)).read()
+        names |= set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(vsz_\w+)\s*\(", header, re.M))
+    assert len(names) == 29
     for n in sorted(names):
         assert hasattr(L, n), n
     assert set(_lib.PROTOTYPES) == names
+    assert "void VapourSynthPluginInit2(" in open(os.path.join(REPO, "include", "vszipcu_vs4.h")).read()
+    assert hasattr(L, "VapourSynthPluginInit2")
 
 
 def _gray(w=32, h=24, n=3, fmt=GRAYS):

@@ -83,6 +83,11 @@ class BatchParams(C.Structure):
     ]
 
 
+class CnnModel(C.Structure):
+    """vsz_cnn_model"""
+    _fields_ = [("w", C.c_void_p * 6), ("b", C.c_void_p * 6)]
+
+
 GET_PLANE_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ssize_t))
 
 # Every function include/vszipcu_bm3d.h declares: name -> (restype, argtypes)
@@ -110,6 +115,19 @@ PROTOTYPES = {
                                      C.c_void_p]),
     "vsz_bm3d_batch_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vsz_bm3d_batch_free": (None, [C.c_void_p]),
+    "vsz_cnn_conv3x3": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "vsz_cnn_conv3x3_dw": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "vsz_cnn_bias_grad": (C.c_int, [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p]),
+    "vsz_cnn_pack_input": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+    "vsz_cnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsz_cnn_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_float,
+                               C.c_float, C.c_float, C.c_int, C.c_void_p]),
+    "vsz_cnn_pack_weights": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsz_cnn_workspace_size": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "vsz_cnn_run": (C.c_int, [C.POINTER(CnnModel), C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsz_filter_frame_range": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vsz_filter_free": (None, [C.c_void_p]),
 }
