@@ -131,3 +131,21 @@ def test_vaggregate_kernel(orc, torch_gpu):
     device.vaggregate(W, r, n, nf, ds, dst)
     torch_gpu.cuda.synchronize()
     assert np.array_equal(dst.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("W,H,rng,metric", [(200, 150, 4, "ssd"), (68, 70, 16, "ssd"), (236, 101, 16, "sad"),
+                                            (132, 90, 30, "ssd"), (320, 240, 12, "sad"), (20, 60, 9, "ssd")])
+def test_shared_column_matcher_block_step4(orc, torch_gpu, W, H, rng, metric):
+    """block_step 4 with (W-8) % 4 == 0 takes the shared-column matcher (columns 0..3 of a block
+    taken from its left neighbour); several 15 x 16 block tiles, partial tiles, clamped rows
+    ((H-8) % 4 != 0), windows larger than the plane."""
+    print(f"shared-column matcher: {W}x{H} R={rng} {metric}", flush=True)
+    torch = torch_gpu
+    img = _noisy(W, H, W * 3 + H)
+    kp, op = kp_and_op(orc, W, H, bs=4, rng=rng, metric=metric, sigma=device.scaled_sigma([10.0] * 3, False))
+    om, oe = orc.match(op, img[None])
+    nbx, nby = device.num_blocks(kp)
+    m = torch.zeros((nby, nbx, 8, 4), dtype=torch.int32, device="cuda")
+    device.match(kp, torch.from_numpy(img[None]).cuda(), m)
+    torch.cuda.synchronize()
+    assert_matches_equal(m.cpu().numpy(), om, oe)

@@ -531,6 +531,134 @@ __global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
     match_lds_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), region);
 }
 
+// ---- shared-column matching for block_step 4 (ssd, sad) ---------------------------------------
+// With block_step 4 the reference block right of x starts at x + 4, so at any displacement its
+// columns 0..3 are the same (reference pixel, candidate pixel) pairs as columns 4..7 of the
+// block at x, and the per-column sums (even/odd row chains, then e0 + e1) are bit-identical.
+// Each lane therefore computes only its columns 4..7 and takes columns 0..3 from the lane on its
+// left with a DPP row shift. Lanes form rows of 16: lane 0 of a row is a helper that computes
+// the columns left of the row's first block (it owns no block), lanes 1..15 own 15 consecutive
+// blocks of one block row. A workgroup covers 15 x 16 blocks. Requires (W - 8) % 4 == 0, so no
+// block position is clamped and neighbours are exactly 4 pixels apart.
+__device__ __forceinline__ float dpp_from_left(float v) {
+    // row_shr:1 within 16-lane rows; lane 0 of a row keeps its own value (bound_ctrl off)
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+
+template <int M, int U>
+__device__ __forceinline__ void lds4_step(float (&w)[5][8], const char *lds, const int (&vo)[8], const float (&a)[32],
+                                          bool ok, unsigned pos, List8 &L) {
+    lds_col<(U + 4) * 4>(w[(U + 4) % 5], lds, vo);
+    float own[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float *b = w[(U + c) % 5];
+        float e0 = 0.0f, e1 = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 8; r += 2) {
+            const float d0 = a[r * 4 + c] - b[r];
+            const float d1 = a[(r + 1) * 4 + c] - b[r + 1];
+            if constexpr (M == 0) {
+                e0 = fmaf_(d0, d0, e0);
+                e1 = fmaf_(d1, d1, e1);
+            } else {
+                e0 = e0 + fabsf(d0);
+                e1 = e1 + fabsf(d1);
+            }
+        }
+        own[c] = e0 + e1;
+    }
+    const float col[8] = {dpp_from_left(own[0]), dpp_from_left(own[1]), dpp_from_left(own[2]), dpp_from_left(own[3]),
+                          own[0], own[1], own[2], own[3]};
+    const float e = tree8(col);
+    if (ok) insert8<false>(L, e, pos, 0);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
+    extern __shared__ float region[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int W = a.width, H = a.height, S = a.stride, Rg = a.bm_range;
+    const int bx0 = tx * 15, by0 = ty * 16;
+    const int x_first = bx0 * 4, x_last = min(bx0 + 14, a.nbx - 1) * 4;
+    const int y_first = min(by0 * 4, H - 8), y_last = min(min(by0 + 15, a.nby - 1) * 4, H - 8);
+    const int rx0 = x_first - Rg, ry0 = y_first - Rg;
+    const int RW = x_last - x_first + 2 * Rg + 9, RH = y_last - y_first + 2 * Rg + 8;
+    const int LS = a.lds_stride;
+
+    for (int i = threadIdx.x; i < RW * RH; i += 256) {
+        const int Y = i / RW, X = i % RW;
+        const int gy = ry0 + Y, gx = rx0 + X;
+        region[lds_row(Y, LS) + X] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? a.img[(long long)gy * S + gx] : 0.0f;
+    }
+    __syncthreads();
+
+    const int rc = (lane & 15) - 1;            // -1: the row's helper lane
+    const int bx = bx0 + rc, by = by0 + wave * 4 + (lane >> 4);
+    const bool valid = rc >= 0 && bx < a.nbx && by < a.nby;
+    const int x = rc < 0 ? x_first - 4 : min(bx, a.nbx - 1) * 4;   // helper: the block left of the row
+    const int y = min(min(by, a.nby - 1) * 4, H - 8);
+    float ref[32];                             // this lane's columns 4..7 (x + 4 >= 0 even for the helper)
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ref[r * 4 + c] = a.img[(long long)(y + r) * S + x + 4 + c];
+    List8 L;
+    list_init<false>(L, 0);
+
+    const char *lds = reinterpret_cast<const char *>(region);
+    const int span = 2 * Rg;
+    const int X0 = x + 4 - Rg - rx0;           // column 4 of the first candidate
+    for (int dy = -Rg; dy <= Rg; ++dy) {
+        const int yy = y + dy;
+        const bool row_ok = valid && yy >= 0 && yy <= H - 8;
+        const unsigned ybits = (unsigned)yy << 16;
+        int vo[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) vo[r] = (lds_row(yy + r - ry0, LS) + X0) * 4;
+        float w[5][8];
+        lds_col<0>(w[0], lds, vo);
+        lds_col<4>(w[1], lds, vo);
+        lds_col<8>(w[2], lds, vo);
+        lds_col<12>(w[3], lds, vo);
+        const int x0 = x - Rg;
+        for (int tb = 0; tb <= span; tb += 5) {
+            const int xt = x0 + tb;
+            lds4_step<M, 0>(w, lds, vo, ref, row_ok && xt >= 0 && xt <= W - 8, (unsigned)xt | ybits, L);
+            if (tb + 1 > span) break;
+            lds4_step<M, 1>(w, lds, vo, ref, row_ok && xt + 1 >= 0 && xt + 1 <= W - 8, (unsigned)(xt + 1) | ybits, L);
+            if (tb + 2 > span) break;
+            lds4_step<M, 2>(w, lds, vo, ref, row_ok && xt + 2 >= 0 && xt + 2 <= W - 8, (unsigned)(xt + 2) | ybits, L);
+            if (tb + 3 > span) break;
+            lds4_step<M, 3>(w, lds, vo, ref, row_ok && xt + 3 >= 0 && xt + 3 <= W - 8, (unsigned)(xt + 3) | ybits, L);
+            if (tb + 4 > span) break;
+            lds4_step<M, 4>(w, lds, vo, ref, row_ok && xt + 4 >= 0 && xt + 4 <= W - 8, (unsigned)(xt + 4) | ybits, L);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) vo[r] += 20;
+        }
+    }
+
+    const unsigned self = pack_xy(x, y);
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) found |= L.pos[k] == self;
+    if (!found) {
+#pragma unroll
+        for (int k = 7; k > 0; --k) {
+            L.er[k] = L.er[k - 1];
+            L.pos[k] = L.pos[k - 1];
+        }
+        L.er[0] = 0.0f;
+        L.pos[0] = self;
+    }
+    if (!valid) return;
+    int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
+}
+
 template <int M, bool TEMPORAL>
 __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
     const int lane = threadIdx.x;
@@ -1171,7 +1299,30 @@ size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride) {
     return bytes <= kMatchLdsBudget ? bytes : 0;
 }
 
+// LDS bytes of the shared-column matcher's 15 x 16 block tile (0: not usable).
+size_t match_lds4_bytes(const MatchArgs &a, int *lds_stride) {
+    if (a.block_step != 4 || (a.width - 8) % 4 != 0) return 0;
+    const long long rw = 14LL * 4 + 2LL * a.bm_range + 9;
+    const long long rh = 15LL * 4 + 2LL * a.bm_range + 8;
+    const long long ls = (rw + 3 + 31) / 32 * 32;
+    const size_t bytes = (size_t)(rh * ls * 4);
+    if (lds_stride) *lds_stride = (int)ls;
+    return bytes <= kMatchLdsBudget ? bytes : 0;
+}
+
 template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s) {
+    if constexpr (M == 0 || M == 1) {
+        if (!temporal) {
+            MatchArgs b = a;
+            const size_t lds = match_lds4_bytes(a, &b.lds_stride);
+            if (lds) {
+                b.tiles_x = (a.nbx + 14) / 15;
+                const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 15) / 16)), 1, 1);
+                hipLaunchKernelGGL((k_match_lds4<M>), grid, dim3(256), lds, s, b);
+                return;
+            }
+        }
+    }
     if (!temporal) {
         MatchArgs b = a;
         const size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride);
