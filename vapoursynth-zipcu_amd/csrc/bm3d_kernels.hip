@@ -422,6 +422,16 @@ template <int IMM> __device__ __forceinline__ void lds_col(float (&col)[8], cons
     for (int r = 0; r < 8; ++r) col[r] = *reinterpret_cast<const float *>(lds + vo[r] + IMM);
 }
 
+// Same, from per-row LDS pointers (the region base is folded in once per search row). The
+// pointers carry the LDS address space so the loads stay ds_read instructions.
+typedef const char __attribute__((address_space(3))) *lds_cptr;
+typedef const float __attribute__((address_space(3))) *lds_fptr;
+
+template <int IMM> __device__ __forceinline__ void lds_colp(float (&col)[8], const lds_cptr (&pr)[8]) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) col[r] = *(lds_fptr)(pr[r] + IMM);
+}
+
 template <int M, int U>
 __device__ __forceinline__ void lds_step(float (&w)[9][8], const char *lds, const int (&vo)[8], int W, int xt,
                                          bool row_ok, unsigned ybits, const RefPatch &R, List8 &L) {
@@ -538,17 +548,20 @@ __global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
 // Each lane therefore computes only its columns 4..7 and takes columns 0..3 from the lane on its
 // left with a DPP row shift. Lanes form rows of 16: lane 0 of a row is a helper that computes
 // the columns left of the row's first block (it owns no block), lanes 1..15 own 15 consecutive
-// blocks of one block row. A workgroup covers 15 x 16 blocks. Requires (W - 8) % 4 == 0, so no
-// block position is clamped and neighbours are exactly 4 pixels apart.
+// blocks of one block row. Each block's search rows are split between two lanes of the same
+// wave (upper and lower half of the window), which doubles the waves in flight; a workgroup
+// covers 15 x 8 blocks. Requires (W - 8) % 4 == 0, so no block position is clamped and
+// neighbours are exactly 4 pixels apart.
 __device__ __forceinline__ float dpp_from_left(float v) {
-    // row_shr:1 within 16-lane rows; lane 0 of a row keeps its own value (bound_ctrl off)
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x111, 0xf, 0xf, false));
+    // row_shr:1 within 16-lane rows; lane 0 of a row (the helper, whose distance is never used)
+    // reads 0 (bound_ctrl), so no copy of the old value is needed
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
 }
 
 template <int M, int U>
-__device__ __forceinline__ void lds4_step(float (&w)[5][8], const char *lds, const int (&vo)[8], const float (&a)[32],
+__device__ __forceinline__ void lds4_step(float (&w)[5][8], const lds_cptr (&pr)[8], const float (&a)[32],
                                           bool ok, unsigned pos, List8 &L) {
-    lds_col<(U + 4) * 4>(w[(U + 4) % 5], lds, vo);
+    lds_colp<(U + 4) * 4>(w[(U + 4) % 5], pr);
     float own[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -581,9 +594,9 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int W = a.width, H = a.height, S = a.stride, Rg = a.bm_range;
-    const int bx0 = tx * 15, by0 = ty * 16;
+    const int bx0 = tx * 15, by0 = ty * 8;
     const int x_first = bx0 * 4, x_last = min(bx0 + 14, a.nbx - 1) * 4;
-    const int y_first = min(by0 * 4, H - 8), y_last = min(min(by0 + 15, a.nby - 1) * 4, H - 8);
+    const int y_first = min(by0 * 4, H - 8), y_last = min(min(by0 + 7, a.nby - 1) * 4, H - 8);
     const int rx0 = x_first - Rg, ry0 = y_first - Rg;
     const int RW = x_last - x_first + 2 * Rg + 9, RH = y_last - y_first + 2 * Rg + 8;
     const int LS = a.lds_stride;
@@ -595,8 +608,11 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
     }
     __syncthreads();
 
+    // DPP row q of the wave: block row by0 + 2*wave + (q & 1), search rows of half q >> 1
+    // (half 0: dy = -R..0, half 1: dy = 1..R); the halves' lists are merged at the end.
+    const int q = lane >> 4, half = q >> 1;
     const int rc = (lane & 15) - 1;            // -1: the row's helper lane
-    const int bx = bx0 + rc, by = by0 + wave * 4 + (lane >> 4);
+    const int bx = bx0 + rc, by = by0 + 2 * wave + (q & 1);
     const bool valid = rc >= 0 && bx < a.nbx && by < a.nby;
     const int x = rc < 0 ? x_first - 4 : min(bx, a.nbx - 1) * 4;   // helper: the block left of the row
     const int y = min(min(by, a.nby - 1) * 4, H - 8);
@@ -608,37 +624,51 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
     List8 L;
     list_init<false>(L, 0);
 
-    const char *lds = reinterpret_cast<const char *>(region);
+    const lds_cptr lds = (lds_cptr)(const char *)region;
     const int span = 2 * Rg;
     const int X0 = x + 4 - Rg - rx0;           // column 4 of the first candidate
-    for (int dy = -Rg; dy <= Rg; ++dy) {
+    const int dy_end = half == 0 ? 0 : Rg;
+    for (int dy = half == 0 ? -Rg : 1; dy <= dy_end; ++dy) {
         const int yy = y + dy;
         const bool row_ok = valid && yy >= 0 && yy <= H - 8;
         const unsigned ybits = (unsigned)yy << 16;
-        int vo[8];
+        lds_cptr pr[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) vo[r] = (lds_row(yy + r - ry0, LS) + X0) * 4;
+        for (int r = 0; r < 8; ++r) pr[r] = lds + (lds_row(yy + r - ry0, LS) + X0) * 4;
         float w[5][8];
-        lds_col<0>(w[0], lds, vo);
-        lds_col<4>(w[1], lds, vo);
-        lds_col<8>(w[2], lds, vo);
-        lds_col<12>(w[3], lds, vo);
+        lds_colp<0>(w[0], pr);
+        lds_colp<4>(w[1], pr);
+        lds_colp<8>(w[2], pr);
+        lds_colp<12>(w[3], pr);
         const int x0 = x - Rg;
         for (int tb = 0; tb <= span; tb += 5) {
             const int xt = x0 + tb;
-            lds4_step<M, 0>(w, lds, vo, ref, row_ok && xt >= 0 && xt <= W - 8, (unsigned)xt | ybits, L);
+            lds4_step<M, 0>(w, pr, ref, row_ok && xt >= 0 && xt <= W - 8, (unsigned)xt | ybits, L);
             if (tb + 1 > span) break;
-            lds4_step<M, 1>(w, lds, vo, ref, row_ok && xt + 1 >= 0 && xt + 1 <= W - 8, (unsigned)(xt + 1) | ybits, L);
+            lds4_step<M, 1>(w, pr, ref, row_ok && xt + 1 >= 0 && xt + 1 <= W - 8, (unsigned)(xt + 1) | ybits, L);
             if (tb + 2 > span) break;
-            lds4_step<M, 2>(w, lds, vo, ref, row_ok && xt + 2 >= 0 && xt + 2 <= W - 8, (unsigned)(xt + 2) | ybits, L);
+            lds4_step<M, 2>(w, pr, ref, row_ok && xt + 2 >= 0 && xt + 2 <= W - 8, (unsigned)(xt + 2) | ybits, L);
             if (tb + 3 > span) break;
-            lds4_step<M, 3>(w, lds, vo, ref, row_ok && xt + 3 >= 0 && xt + 3 <= W - 8, (unsigned)(xt + 3) | ybits, L);
+            lds4_step<M, 3>(w, pr, ref, row_ok && xt + 3 >= 0 && xt + 3 <= W - 8, (unsigned)(xt + 3) | ybits, L);
             if (tb + 4 > span) break;
-            lds4_step<M, 4>(w, lds, vo, ref, row_ok && xt + 4 >= 0 && xt + 4 <= W - 8, (unsigned)(xt + 4) | ybits, L);
+            lds4_step<M, 4>(w, pr, ref, row_ok && xt + 4 >= 0 && xt + 4 <= W - 8, (unsigned)(xt + 4) | ybits, L);
 #pragma unroll
-            for (int r = 0; r < 8; ++r) vo[r] += 20;
+            for (int r = 0; r < 8; ++r) pr[r] += 20;
         }
     }
+
+    // Merge: half 1 scanned later rows, so inserting its entries in order into half 0's list
+    // with the same strict-< rule gives exactly the list of one sequential scan.
+    float oe[8];
+    unsigned op[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        oe[k] = __shfl_xor(L.er[k], 32);
+        op[k] = (unsigned)__shfl_xor((int)L.pos[k], 32);
+    }
+    if (half != 0 || !valid) return;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) insert8<false>(L, oe[k], op[k], 0);
 
     const unsigned self = pack_xy(x, y);
     bool found = false;
@@ -653,7 +683,6 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
         L.er[0] = 0.0f;
         L.pos[0] = self;
     }
-    if (!valid) return;
     int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
@@ -1303,7 +1332,7 @@ size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride) {
 size_t match_lds4_bytes(const MatchArgs &a, int *lds_stride) {
     if (a.block_step != 4 || (a.width - 8) % 4 != 0) return 0;
     const long long rw = 14LL * 4 + 2LL * a.bm_range + 9;
-    const long long rh = 15LL * 4 + 2LL * a.bm_range + 8;
+    const long long rh = 7LL * 4 + 2LL * a.bm_range + 8;
     const long long ls = (rw + 3 + 31) / 32 * 32;
     const size_t bytes = (size_t)(rh * ls * 4);
     if (lds_stride) *lds_stride = (int)ls;
@@ -1317,7 +1346,7 @@ template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStrea
             const size_t lds = match_lds4_bytes(a, &b.lds_stride);
             if (lds) {
                 b.tiles_x = (a.nbx + 14) / 15;
-                const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 15) / 16)), 1, 1);
+                const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 7) / 8)), 1, 1);
                 hipLaunchKernelGGL((k_match_lds4<M>), grid, dim3(256), lds, s, b);
                 return;
             }
