@@ -588,10 +588,8 @@ __device__ __forceinline__ void lds4_step(float (&w)[5][8], const lds_cptr (&pr)
 }
 
 template <int M>
-__global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
-    extern __shared__ float region[];
+__device__ __forceinline__ void match_lds4_body(const MatchArgs &a, int tile, float *region) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int W = a.width, H = a.height, S = a.stride, Rg = a.bm_range;
     const int bx0 = tx * 15, by0 = ty * 8;
@@ -686,6 +684,12 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
     int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
+    extern __shared__ float region[];
+    match_lds4_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), region);
 }
 
 template <int M, bool TEMPORAL>
@@ -1251,6 +1255,15 @@ __global__ __launch_bounds__(256) void k_match_lds_batch(const BatchTable *__res
     match_lds_body<M>(a, g - t->match_off[p], region);
 }
 
+template <int M>
+__global__ __launch_bounds__(256) void k_match_lds4_batch(const BatchTable *__restrict__ t) {
+    extern __shared__ float region[];
+    const int g = xcd_tile(blockIdx.x, gridDim.x);
+    const int p = batch_plane(t->match_off, t->n, g);
+    const MatchArgs &a = t->m[p];
+    match_lds4_body<M>(a, g - t->match_off[p], region);
+}
+
 template <bool FINAL, int T2, int T1, bool EXACT>
 __global__ __launch_bounds__(256, 2) void k_filter_tile_batch(const BatchTable *__restrict__ t) {
     extern __shared__ float smem[];
@@ -1661,7 +1674,12 @@ void launch_filter_batch(const BatchTable *d, int wgs, size_t lds, hipStream_t s
     hipLaunchKernelGGL((k_filter_tile_batch<FINAL, T, T, EXACT>), dim3(wgs), dim3(256), lds, s, d);
 }
 
-void launch_match_batch(int metric, const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
+void launch_match_batch(int metric, bool shared_cols, const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
+    if (shared_cols) {
+        if (metric == 0) hipLaunchKernelGGL((k_match_lds4_batch<0>), dim3(wgs), dim3(256), lds, s, d);
+        else hipLaunchKernelGGL((k_match_lds4_batch<1>), dim3(wgs), dim3(256), lds, s, d);
+        return;
+    }
     switch (metric) {
     case 0: hipLaunchKernelGGL((k_match_lds_batch<0>), dim3(wgs), dim3(256), lds, s, d); break;
     case 1: hipLaunchKernelGGL((k_match_lds_batch<1>), dim3(wgs), dim3(256), lds, s, d); break;
@@ -1680,6 +1698,7 @@ struct vsz_bm3d_batch {
     size_t acc_total = 0, work_bytes = 0;
     size_t match_lds = 0;
     int match_stride = 0;
+    bool shared_cols = false;  // every plane takes the shared-column block_step-4 matcher
     bool tuned = false;
     BatchTable *h_tab[2] = {nullptr, nullptr};
     BatchTable *d_tab = nullptr;  // [2]
@@ -1711,9 +1730,11 @@ struct vsz_bm3d_batch {
         for (const auto &pl : planes) m = std::max(m, pl.tile[pass].lds_bytes);
         return m;
     }
+    int match_tiles_x(const BatchPlane &pl) const { return shared_cols ? (pl.L.nbx + 14) / 15 : (pl.L.nbx + 15) / 16; }
+    int match_tiles_y(const BatchPlane &pl) const { return shared_cols ? (pl.L.nby + 7) / 8 : (pl.L.nby + 15) / 16; }
     int match_wgs() const {
         int n = 0;
-        for (const auto &pl : planes) n += ((pl.L.nbx + 15) / 16) * ((pl.L.nby + 15) / 16);
+        for (const auto &pl : planes) n += match_tiles_x(pl) * match_tiles_y(pl);
         return n;
     }
     int finish_wgs() const {
@@ -1752,9 +1773,9 @@ struct vsz_bm3d_batch {
                 m.ps_num = k.ps_num;
                 m.ps_range = k.ps_range;
                 m.ts = pl.L.ts;
-                m.tiles_x = (pl.L.nbx + 15) / 16;
+                m.tiles_x = match_tiles_x(pl);
                 m.lds_stride = match_stride;
-                mo += m.tiles_x * ((pl.L.nby + 15) / 16);
+                mo += m.tiles_x * match_tiles_y(pl);
                 FilterArgs &f = t.f[i];
                 f = FilterArgs{};
                 f.matches = matches;
@@ -1812,7 +1833,7 @@ struct vsz_bm3d_batch {
         for (int pass = 0; pass < passes(); ++pass) {
             const BatchTable *d = d_tab + pass;
             hipLaunchKernelGGL(k_zero, dim3(zgrid), dim3(256), 0, s, (unsigned long long *)work, n64);
-            launch_match_batch(bp.metric, d, match_wgs(), match_lds, s);
+            launch_match_batch(bp.metric, shared_cols, d, match_wgs(), match_lds, s);
             const int fw = filter_wgs(pass);
             const size_t fl = filter_lds(pass);
             const bool mf = bp.transform_precision == 1;
@@ -1933,6 +1954,23 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
     b->match_stride = ls;
     if (!b->match_lds)
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: block_step/bm_range too large for the LDS-staged matcher");
+    // the shared-column matcher when every plane allows it (block_step 4, (W-8) % 4 == 0, ssd/sad)
+    b->shared_cols = bp->block_step == 4 && (bp->metric == 0 || bp->metric == 1);
+    for (int i = 0; i < n && b->shared_cols; ++i)
+        if ((planes[i].width - 8) % 4 != 0) b->shared_cols = false;
+    if (b->shared_cols) {
+        MatchArgs probe{};
+        probe.block_step = 4;
+        probe.width = 8;
+        probe.bm_range = bp->bm_range;
+        const size_t l4 = match_lds4_bytes(probe, &ls);
+        if (l4) {
+            b->match_lds = l4;
+            b->match_stride = ls;
+        } else {
+            b->shared_cols = false;
+        }
+    }
     const float fac[2] = {bits_float(0x3f021bb6u), bits_float(0x3e40c0c1u)};  // src/bm3d.zig:1013-1024
     size_t acc = 0, rest = 0;
     for (int i = 0; i < n; ++i) {
