@@ -1020,17 +1020,20 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
         float r[FINAL ? 64 : 1];
         gather_group<FINAL>(a, p, mx, my, mz, c, v, r);
         const float w = filter_group<FINAL, T2, T1>(v, r, a.sigma[p], buf, c);
+        // back to column layout, so the 8 lanes of a group update 8 adjacent words of one row
+        // with each atomic instruction (one cache line per group instead of eight)
+        transpose_ranks(v, buf, c);
         if (!valid) continue;
         const long long pb = (long long)p * a.acc_plane_elems;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {  // row layout: this lane adds row (lane & 7) of patch i
-            const long long base = pb + (long long)mz[i] * 2 * a.ts + (long long)(my[i] + c) * a.stride + mx[i];
+        for (int i = 0; i < 8; ++i) {  // column layout: this lane adds column (lane & 7) of patch i
+            const long long base = pb + (long long)mz[i] * 2 * a.ts + (long long)my[i] * a.stride + mx[i] + c;
 #pragma unroll
-            for (int cc = 0; cc < 8; ++cc) {
-                const float q = (w * v[i * 8 + cc] + E) - E;
-                if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + base + cc,
+            for (int j = 0; j < 8; ++j) {
+                const float q = (w * v[i * 8 + j] + E) - E;
+                if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + base + (long long)j * a.stride,
                                                (unsigned long long)to_fixed(q, a.extractor_exp));
-                else atomicAdd((float *)a.acc + base + cc, q);
+                else atomicAdd((float *)a.acc + base + (long long)j * a.stride, q);
             }
         }
         if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
