@@ -517,7 +517,7 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
     }
 
     const unsigned self = pack_xy(x, y);
-    bool found = false;
+    bool found = a.no_self != 0;   // a temporal stage does the self-inclusion after its merges
 #pragma unroll
     for (int k = 0; k < 8; ++k) found |= L.pos[k] == self;
     if (!found) {
@@ -669,7 +669,7 @@ __device__ __forceinline__ void match_lds4_body(const MatchArgs &a, int tile, fl
     for (int k = 0; k < 8; ++k) insert8<false>(L, oe[k], op[k], 0);
 
     const unsigned self = pack_xy(x, y);
-    bool found = false;
+    bool found = a.no_self != 0;   // a temporal stage does the self-inclusion after its merges
 #pragma unroll
     for (int k = 0; k < 8; ++k) found |= L.pos[k] == self;
     if (!found) {
@@ -765,6 +765,83 @@ __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), TEMPORAL ? L.zs[k] : 0, __float_as_int(L.er[k]));
+}
+
+// Temporal stage of a radius > 0 pass whose centre-frame spatial lists were written by an
+// LDS-staged matcher (without self-inclusion): the predictive search in the other frames, the
+// merges and the self-inclusion, exactly as the single-kernel path above does after its spatial
+// scan (src/bm3d.cu:663-788).
+template <int M>
+__global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
+    const bool valid = bx < a.nbx && by < a.nby;
+    const int W = a.width, H = a.height, S = a.stride, Rt = a.radius;
+    const int x = min(min(bx, a.nbx - 1) * a.block_step, W - 8);
+    const int y = min(min(by, a.nby - 1) * a.block_step, H - 8);
+    const float *centre = a.img + (long long)Rt * a.ts;
+    const PlaneBuf pb = make_plane_buf(a.img, (unsigned)(a.ts * (2 * Rt + 1) * 4));
+
+    RefPatch R;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) R.a[r * 8 + c] = centre[(long long)(y + r) * S + x + c];
+    ref_terms<M>(R);
+
+    int4 *out = a.matches + ((long long)min(by, a.nby - 1) * a.nbx + min(bx, a.nbx - 1)) * 8;
+    List8 L;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int4 m = out[k];
+        L.er[k] = __int_as_float(m.w);
+        L.pos[k] = pack_xy(m.x, m.y);
+        L.zs[k] = Rt;
+    }
+    unsigned cpos[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cpos[k] = L.pos[k];
+    for (int dir = -1; dir <= 1; dir += 2) {
+        unsigned lpos[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) lpos[k] = cpos[k];
+        for (int t = 1; t <= Rt; ++t) {
+            const int tz = Rt + dir * t;
+            List8 F;
+            list_init<false>(F, tz);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (i < a.ps_num)
+                    scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
+                                          a.ps_range, tz, F);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (i < a.ps_num) insert8<true>(L, F.er[i], F.pos[i], tz);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) lpos[k] = F.pos[k];
+        }
+    }
+
+    const unsigned self = pack_xy(x, y);
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) found |= (L.pos[k] == self && L.zs[k] == Rt);
+    if (!found) {
+#pragma unroll
+        for (int k = 7; k > 0; --k) {
+            L.er[k] = L.er[k - 1];
+            L.pos[k] = L.pos[k - 1];
+            L.zs[k] = L.zs[k - 1];
+        }
+        L.er[0] = 0.0f;
+        L.pos[0] = self;
+        L.zs[0] = Rt;
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), L.zs[k], __float_as_int(L.er[k]));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1355,7 +1432,25 @@ size_t match_lds4_bytes(const MatchArgs &a, int *lds_stride) {
     return bytes <= kMatchLdsBudget ? bytes : 0;
 }
 
+template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s);
+
+// Radius > 0: the centre frame's spatial search on an LDS-staged matcher, then the temporal
+// stage; false when neither LDS matcher fits (the single-kernel path then does everything).
+template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
+    MatchArgs c = a;
+    c.img = a.img + (long long)a.radius * a.ts;
+    c.no_self = 1;
+    int ls = 0;
+    const bool lds4 = (M == 0 || M == 1) && match_lds4_bytes(a, &ls) != 0;
+    if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls)) return false;
+    launch_match_m<M>(c, false, s);
+    const dim3 grid((unsigned)(a.tiles_x * ((a.nby + 7) / 8)), 1, 1);
+    hipLaunchKernelGGL((k_match_temporal<M>), grid, dim3(64), 0, s, a);
+    return true;
+}
+
 template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s) {
+    if (temporal && launch_match_split<M>(a, s)) return;
     if constexpr (M == 0 || M == 1) {
         if (!temporal) {
             MatchArgs b = a;

@@ -31,6 +31,7 @@ struct MatchArgs {
     long long ts;
     int tiles_x;     // block tiles per row
     int lds_stride;  // LDS-staged path: floats per staged row
+    int no_self;     // 1: leave out the self-inclusion (a temporal stage follows)
 };
 
 struct FilterArgs {
