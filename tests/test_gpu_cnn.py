@@ -146,3 +146,39 @@ def test_committed_weights_denoise(torch_gpu):
     p_noisy, p_out = synth.psnr(nz, clean), synth.psnr(out, clean)
     print(f"  PSNR noisy {p_noisy:.2f} dB, learned basic {p_out:.2f} dB", flush=True)
     assert p_out > p_noisy + 3.0
+
+
+def test_inference_path_matches_training_forward(torch_gpu):
+    """vsz_cnn_run (first/last layers on FP32 VALU, middle layers on MFMA) against the forward pass
+    of the training kernels (every layer on the padded MFMA kernel) on the same weights: the
+    same bf16 operands, different summation order, so bf16 activations can round the other way
+    and the outputs agree to that rounding: max-abs <= 1e-2 and mean-abs <= 1e-3 of the output
+    range (the output layer is scaled up here so that it carries real weight; a layout error
+    gives differences of the order of the range itself)."""
+    torch = torch_gpu
+    L = _lib.load()
+    m = cnn.Model.init(11)
+    for li in range(6):   # non-trivial biases and a real output layer
+        m.b[li].copy_(torch.linspace(-0.05, 0.05, CH, device="cuda"))
+        m.w[li].mul_(1.0 if li < 5 else 100.0)
+    m.pack()
+    n, h, w = 2, 40, 52
+    clean = np.stack([synth.clean_luma(w, h, 5 + i) for i in range(n)])
+    nz = torch.from_numpy(np.stack([synth.noisy(clean[i], 10.0 + 5 * i, 9 + i) for i in range(n)])).cuda()
+    got = m.run(nz, [10.0, 15.0])
+    tr = cnn.Trainer(m, n, h, w)
+    sig = torch.tensor([10.0 / 255, 15.0 / 255], device="cuda")
+    s = cnn._s(None)
+    cnn._check(L.vsz_cnn_pack_input(cnn._p(nz), n, h, w, w, cnn._p(sig), cnn._p(tr.a[0]), cnn._p(tr.base), s))
+    for li in range(5):
+        cnn._check(L.vsz_cnn_conv3x3(cnn._p(tr.a[li]), cnn._p(m.wf[li]), cnn._p(m.b[li]), n, h, w, 0, None, None,
+                                     cnn._p(tr.a[li + 1]), 0, s))
+    cnn._check(L.vsz_cnn_conv3x3(cnn._p(tr.a[5]), cnn._p(m.wf[5]), cnn._p(m.b[5]), n, h, w, 1, None, cnn._p(tr.base),
+                                 cnn._p(tr.out), w, s))
+    torch.cuda.synchronize()
+    ref = tr.out.view(n, h, w)
+    d = (got - ref).abs()
+    scale = float(ref.abs().max())
+    print(f"  inference vs training forward: max-abs {float(d.max()):.3g}, mean-abs {float(d.mean()):.3g}, "
+          f"output range {scale:.3g}", flush=True)
+    assert float(d.max()) <= 1e-2 * scale and float(d.mean()) <= 1e-3 * scale

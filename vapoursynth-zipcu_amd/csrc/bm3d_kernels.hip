@@ -1229,7 +1229,8 @@ __global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
 // ------------------------------------------------------------------------------------------------
 template <bool EXACT>
 __device__ __forceinline__ void finish_body(const FinishArgs &a, int tile_x, int tile_y, int pz,
-                                            typename std::conditional<EXACT, long long, float>::type (*sD)[72]) {
+                                            typename std::conditional<EXACT, long long, float>::type (*sD)[72],
+                                            typename std::conditional<EXACT, long long, float>::type (*sH)[64]) {
     using T = typename std::conditional<EXACT, long long, float>::type;
     const int x0 = tile_x * 64, y0 = tile_y * 16;
     const int p = pz / a.tw;
@@ -1243,6 +1244,15 @@ __device__ __forceinline__ void finish_body(const FinishArgs &a, int tile_x, int
         sD[rr][cc] = (y >= 0 && y < a.height && x >= 0 && x < a.width) ? D[(long long)y * a.stride + x] : T(0);
     }
     __syncthreads();
+    // separable 8x8 box sum: horizontal 8-sums of the 23 staged rows, then vertical 8-sums
+    for (int idx = threadIdx.x; idx < 23 * 64; idx += 256) {
+        const int rr = idx >> 6, cc = idx & 63;
+        T h = T(0);
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) h += sD[rr][cc + dx];
+        sH[rr][cc] = h;
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int px = threadIdx.x & 63, py = (threadIdx.x >> 6) + 4 * k;
@@ -1250,9 +1260,7 @@ __device__ __forceinline__ void finish_body(const FinishArgs &a, int tile_x, int
         if (x >= a.width || y >= a.height) continue;
         T s = T(0);
 #pragma unroll
-        for (int dy = 0; dy < 8; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 8; ++dx) s += sD[py + dy][px + dx];
+        for (int dy = 0; dy < 8; ++dy) s += sH[py + dy][px];
         const long long i = (long long)y * a.stride + x;
         float fn, fd;
         if constexpr (EXACT) {
@@ -1274,7 +1282,8 @@ __device__ __forceinline__ void finish_body(const FinishArgs &a, int tile_x, int
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_finish(FinishArgs a) {
     __shared__ typename std::conditional<EXACT, long long, float>::type sD[23][72];
-    finish_body<EXACT>(a, blockIdx.x, blockIdx.y, blockIdx.z, sD);
+    __shared__ typename std::conditional<EXACT, long long, float>::type sH[23][64];
+    finish_body<EXACT>(a, blockIdx.x, blockIdx.y, blockIdx.z, sD, sH);
 }
 
 __global__ __launch_bounds__(256) void k_zero(unsigned long long *p, long long n64) {
@@ -1357,11 +1366,12 @@ __global__ __launch_bounds__(256, 2) void k_filter_tile_batch(const BatchTable *
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_finish_batch(const BatchTable *__restrict__ t) {
     __shared__ typename std::conditional<EXACT, long long, float>::type sD[23][72];
+    __shared__ typename std::conditional<EXACT, long long, float>::type sH[23][64];
     const int g = blockIdx.x;
     const int p = batch_plane(t->finish_off, t->n, g);
     const FinishArgs &a = t->fin[p];
     const int local = g - t->finish_off[p], tx = t->finish_tiles_x[p], ty = t->finish_tiles_y[p];
-    finish_body<EXACT>(a, local % tx, (local / tx) % ty, local / (tx * ty), sD);
+    finish_body<EXACT>(a, local % tx, (local / tx) % ty, local / (tx * ty), sD, sH);
 }
 
 }  // namespace vszk

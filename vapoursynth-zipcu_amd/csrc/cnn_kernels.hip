@@ -278,6 +278,100 @@ __global__ __launch_bounds__(256) void k_pack_weights(const float *__restrict__ 
     if (wb) wb[((8 - t) * CH + in) * CH + o] = u;
 }
 
+// ---- inference ends: first layer (2 -> 32) and output layer (32 -> 1) on FP32 VALU -----------
+// Inputs and weights are the same bf16 values the MFMA layers would see (products of bf16
+// values are exact in FP32); only the summation order differs. One thread per output pixel of a
+// 16 x 16 tile; the weights are wave-uniform, so they stay in scalar registers.
+
+// x1[n][y][x][o] = relu(b[o] + sum_t w[t][o][0] * bf16(noisy(p + t)) + w[t][o][1] * bf16(sigma) [p + t inside])
+// w0: fp32 [9][2][32] holding bf16-rounded values.
+__global__ __launch_bounds__(256) void k_conv_first(const float *__restrict__ noisy, int H, int W, int stride,
+                                                    const float *__restrict__ sigma_map, const float *__restrict__ w0,
+                                                    const float *__restrict__ b0, unsigned short *__restrict__ y) {
+    __shared__ float tile[HALO][HALO + 1];
+    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE, n = blockIdx.z;
+    for (int i = threadIdx.x; i < HALO * HALO; i += 256) {
+        const int hy = i / HALO, hx = i % HALO;
+        const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+        // outside the image: a NaN marker, so the sigma term can be dropped there as well
+        tile[hy][hx] = (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                           ? (float)(__bf16)noisy[((long long)n * H + gy) * stride + gx]
+                           : __builtin_nanf("");
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = x0 + tx, gy = y0 + ty;
+    const float sg = (float)(__bf16)sigma_map[n];
+    float acc[CH];
+#pragma unroll
+    for (int o = 0; o < CH; ++o) acc[o] = b0[o];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const float v = tile[ty + t / 3][tx + t % 3];
+        const bool in = v == v;
+        const float xv = in ? v : 0.0f, sv = in ? sg : 0.0f;
+#pragma unroll
+        for (int o = 0; o < CH; ++o) {
+            acc[o] = fmaf(w0[(t * 2 + 0) * CH + o], xv, acc[o]);
+            acc[o] = fmaf(w0[(t * 2 + 1) * CH + o], sv, acc[o]);
+        }
+    }
+    if (gx >= W || gy >= H) return;
+    __bf16 h[CH];
+#pragma unroll
+    for (int o = 0; o < CH; ++o) h[o] = (__bf16)fmaxf(acc[o], 0.0f);
+    uint4 *dst = reinterpret_cast<uint4 *>(y + (((long long)n * H + gy) * W + gx) * CH);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dst[k] = reinterpret_cast<const uint4 *>(h)[k];
+}
+
+// out[n][y][x] (row stride ostride) = noisy[n][y][x] + b5[0] + sum_t sum_i w5[t][i] * x[p + t][i]
+// w5: fp32 [9][32] holding bf16-rounded values; noisy: fp32 [n][H][stride].
+__global__ __launch_bounds__(256) void k_conv_last(const unsigned short *__restrict__ x, int H, int W,
+                                                   const float *__restrict__ w5, const float *__restrict__ b5,
+                                                   const float *__restrict__ noisy, int stride,
+                                                   float *__restrict__ out, int ostride) {
+    __shared__ __attribute__((aligned(16))) char lds[HALO * HALO * PIX_BYTES];
+    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE, n = blockIdx.z;
+    stage_halo(x, n, H, W, x0, y0, lds);
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx = x0 + tx, gy = y0 + ty;
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const char *px = lds + ((ty + t / 3) * HALO + tx + t % 3) * PIX_BYTES;
+#pragma unroll
+        for (int part = 0; part < 4; ++part) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(px + part * 16);
+            const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = part * 8 + 2 * k;
+                acc = fmaf(w5[t * CH + i], __uint_as_float(u[k] << 16), acc);
+                acc = fmaf(w5[t * CH + i + 1], __uint_as_float(u[k] & 0xffff0000u), acc);
+            }
+        }
+    }
+    if (gx >= W || gy >= H) return;
+    out[((long long)n * H + gy) * ostride + gx] = noisy[((long long)n * H + gy) * stride + gx] + (acc + b5[0]);
+}
+
+// fp32 copies (bf16-rounded) of the end layers' weights from the packed forward layout.
+__global__ __launch_bounds__(256) void k_end_weights(const unsigned short *__restrict__ wf0,
+                                                     const unsigned short *__restrict__ wf5, float *__restrict__ w0,
+                                                     float *__restrict__ w5) {
+    const int i = threadIdx.x + blockIdx.x * 256;
+    if (i < 9 * 2 * CH) {  // w0[t][i][o] = wf0[t][o][i], i in {0, 1}
+        const int t = i / (2 * CH), ii = (i / CH) % 2, o = i % CH;
+        w0[i] = bf2f(wf0[(t * CH + o) * CH + ii]);
+    }
+    if (i < 9 * CH) {      // w5[t][i] = wf5[t][0][i]
+        const int t = i / CH, ii = i % CH;
+        w5[i] = bf2f(wf5[(t * CH + 0) * CH + ii]);
+    }
+}
+
 }  // namespace vszcnn
 
 using namespace vszcnn;
@@ -306,24 +400,34 @@ int vsz_cnn_conv3x3(const void *x, const void *w, const float *bias, int n, int 
 size_t vsz_cnn_workspace_size(int n, int h, int wd) {
     if (n < 1 || h < 1 || wd < 1) return 0;
     const size_t px = (size_t)n * h * wd;
-    return 2 * px * CH * 2 + ((px * 4 + 255) & ~size_t(255));
+    return 2 * px * CH * 2 + 4096;   // two activation buffers + the end layers' fp32 weights
 }
 
 int vsz_cnn_run(const vsz_cnn_model *m, const float *noisy, int n, int h, int wd, int stride, const float *sigma_map,
                 float *out, void *work, void *stream) {
-    if (!m || !noisy || !sigma_map || !out || !work || n < 1 || h < 1 || wd < 1 || stride < wd)
+    if (!m || !noisy || !sigma_map || !out || !work || n < 1 || n > 65535 || h < 1 || wd < 1 || stride < wd ||
+        (h + TILE - 1) / TILE > 65535)
         return vsz_fail(VSZ_EINVAL, "vsz_cnn_run: invalid arguments");
     for (int l = 0; l < 6; ++l)
         if (!m->w[l] || !m->b[l]) return vsz_fail(VSZ_EINVAL, "vsz_cnn_run: missing layer %d", l);
+    hipStream_t s = (hipStream_t)stream;
     const size_t px = (size_t)n * h * wd;
     char *wk = static_cast<char *>(work);
     void *act[2] = {wk, wk + px * CH * 2};
-    float *base = reinterpret_cast<float *>(wk + 2 * px * CH * 2);
-    int rc = vsz_cnn_pack_input(noisy, n, h, wd, stride, sigma_map, act[0], base, stream);
-    for (int l = 0; l < 5 && rc == VSZ_OK; ++l)
-        rc = vsz_cnn_conv3x3(act[l & 1], m->w[l], m->b[l], n, h, wd, 0, nullptr, nullptr, act[(l + 1) & 1], 0, stream);
-    if (rc == VSZ_OK) rc = vsz_cnn_conv3x3(act[1], m->w[5], m->b[5], n, h, wd, 1, nullptr, base, out, stride, stream);
-    return rc;
+    float *w0 = reinterpret_cast<float *>(wk + 2 * px * CH * 2);   // [9][2][32]
+    float *w5 = w0 + 9 * 2 * CH;                                     // [9][32]
+    hipLaunchKernelGGL(k_end_weights, dim3(3), dim3(256), 0, s, (const unsigned short *)m->w[0],
+                       (const unsigned short *)m->w[5], w0, w5);
+    const dim3 grid((wd + TILE - 1) / TILE, (h + TILE - 1) / TILE, n);
+    hipLaunchKernelGGL(k_conv_first, grid, dim3(256), 0, s, noisy, h, wd, stride, sigma_map, w0, m->b[0],
+                       (unsigned short *)act[0]);
+    int rc = launch_ok("vsz_cnn_run: first layer");
+    for (int l = 1; l < 5 && rc == VSZ_OK; ++l)
+        rc = vsz_cnn_conv3x3(act[(l - 1) & 1], m->w[l], m->b[l], n, h, wd, 0, nullptr, nullptr, act[l & 1], 0, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_conv_last, grid, dim3(256), 0, s, (const unsigned short *)act[0], h, wd, w5, m->b[5], noisy,
+                       stride, out, stride);
+    return launch_ok("vsz_cnn_run: output layer");
 }
 
 int vsz_cnn_conv3x3_dw(const void *x, const void *dy, int n, int h, int wd, float *dw, void *stream) {
