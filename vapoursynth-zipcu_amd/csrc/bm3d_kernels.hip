@@ -374,14 +374,30 @@ __device__ __forceinline__ void scan_window(const PlaneBuf &b, long long frame_o
         int vo[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) vo[r] = (int)((frame_off + (long long)(yy + r) * S + x0) * 4);
-        // first 8 columns: separately clamped offsets, so no merged vector access can straddle
-        // the buffer start (a partially out-of-range vector load returns all zeros)
+        // first 8 columns. A window starting inside the row loads them as two 16-byte vectors per
+        // row (if the first candidate is valid its 8 columns lie inside the row; if not, nothing
+        // in this row is inserted). A window starting left of the row uses separately clamped
+        // scalar offsets, so no vector access can straddle the buffer start (a partially
+        // out-of-range vector load returns all zeros).
         float w[9][8];
+        if (x0 >= 0) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int dj = 4 * (max(x0 + j, 0) - x0);
+            for (int r = 0; r < 8; ++r)
 #pragma unroll
-            for (int r = 0; r < 8; ++r) w[j][r] = buf_load<0>(b, vo[r] + dj);
+                for (int q = 0; q < 2; ++q) {
+                    const int4 v = __builtin_bit_cast(int4, __builtin_amdgcn_raw_buffer_load_b128(b.rsrc, vo[r] + 16 * q, 0, 0));
+                    w[4 * q + 0][r] = __int_as_float(v.x);
+                    w[4 * q + 1][r] = __int_as_float(v.y);
+                    w[4 * q + 2][r] = __int_as_float(v.z);
+                    w[4 * q + 3][r] = __int_as_float(v.w);
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int dj = 4 * (max(x0 + j, 0) - x0);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) w[j][r] = buf_load<0>(b, vo[r] + dj);
+            }
         }
         for (int tb = 0; tb <= span; tb += 9) {
             const int xt = x0 + tb;
