@@ -149,3 +149,30 @@ def test_shared_column_matcher_block_step4(orc, torch_gpu, W, H, rng, metric):
     device.match(kp, torch.from_numpy(img[None]).cuda(), m)
     torch.cuda.synchronize()
     assert_matches_equal(m.cpu().numpy(), om, oe)
+
+
+@pytest.mark.parametrize("W,H,bs,rng,metric", [(200, 150, 8, 9, "ssd"), (333, 121, 8, 9, "zsad"), (140, 260, 7, 12, "ssd")])
+def test_lds_matcher_short_tiles(orc, torch_gpu, W, H, bs, rng, metric):
+    """Block steps whose 16 x 16-block search region exceeds the LDS budget (the plugin default
+    block_step 8 with bm_range 9) take the 16 x 8-block tile of the LDS-staged matcher."""
+    print(f"short-tile matcher: {W}x{H} bs={bs} R={rng} {metric}", flush=True)
+    torch = torch_gpu
+    img = _noisy(W, H, W + 5 * H)
+    kp, op = kp_and_op(orc, W, H, bs=bs, rng=rng, metric=metric, sigma=device.scaled_sigma([10.0] * 3, False))
+    om, oe = orc.match(op, img[None])
+    nbx, nby = device.num_blocks(kp)
+    m = torch.zeros((nby, nbx, 8, 4), dtype=torch.int32, device="cuda")
+    device.match(kp, torch.from_numpy(img[None]).cuda(), m)
+    torch.cuda.synchronize()
+    assert_matches_equal(m.cpu().numpy(), om, oe)
+
+
+@pytest.mark.parametrize("bs,rng", [(8, 9), (4, 16)])
+def test_temporal_split_matcher(orc, torch_gpu, bs, rng):
+    """radius > 0: centre-frame search on an LDS-staged matcher, then the predictive search."""
+    print(f"temporal split: bs={bs} R={rng}", flush=True)
+    _, seq = synth.moving_sequence(72, 64, 5, 8.0, seed=41)
+    src = np.stack(seq)[None, None]                            # [1, 1, 5, H, W]
+    kp, op = kp_and_op(orc, 72, 64, bs=bs, rng=rng, radius=2, ps_num=2, ps_range=3,
+                       sigma=device.scaled_sigma([8.0] * 3, False))
+    _check(orc, torch_gpu, kp, op, src)

@@ -456,19 +456,20 @@ __device__ __forceinline__ void lds_step(float (&w)[9][8], const char *lds, cons
     if (row_ok && xt >= 0 && xt <= W - 8) insert8<false>(L, e, (unsigned)xt | ybits, 0);
 }
 
-template <int M>
+// WR wave rows: a workgroup of 128 * WR lanes covers 16 x 8*WR blocks.
+template <int M, int WR = 2>
 __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, float *region) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int W = a.width, H = a.height, S = a.stride, bs = a.block_step, Rg = a.bm_range;
-    const int bx0 = tx * 16, by0 = ty * 16;
+    const int bx0 = tx * 16, by0 = ty * (8 * WR);
     const int x_first = min(bx0 * bs, W - 8), x_last = min(min(bx0 + 15, a.nbx - 1) * bs, W - 8);
-    const int y_first = min(by0 * bs, H - 8), y_last = min(min(by0 + 15, a.nby - 1) * bs, H - 8);
+    const int y_first = min(by0 * bs, H - 8), y_last = min(min(by0 + 8 * WR - 1, a.nby - 1) * bs, H - 8);
     const int rx0 = x_first - Rg, ry0 = y_first - Rg;
     const int RW = x_last - x_first + 2 * Rg + 9, RH = y_last - y_first + 2 * Rg + 8;
     const int LS = a.lds_stride;
 
-    for (int i = threadIdx.x; i < RW * RH; i += 256) {
+    for (int i = threadIdx.x; i < RW * RH; i += 128 * WR) {
         const int Y = i / RW, X = i % RW;
         const int gy = ry0 + Y, gx = rx0 + X;
         region[lds_row(Y, LS) + X] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? a.img[(long long)gy * S + gx] : 0.0f;
@@ -551,10 +552,10 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
 }
 
-template <int M>
-__global__ __launch_bounds__(256) void k_match_lds(MatchArgs a) {
+template <int M, int WR>
+__global__ __launch_bounds__(128 * WR) void k_match_lds(MatchArgs a) {
     extern __shared__ float region[];
-    match_lds_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), region);
+    match_lds_body<M, WR>(a, xcd_tile(blockIdx.x, gridDim.x), region);
 }
 
 // ---- shared-column matching for block_step 4 (ssd, sad) ---------------------------------------
@@ -1435,12 +1436,12 @@ Layout layout_of(const vsz_bm3d_kparams *p) {
     return L;
 }
 
-// LDS bytes the staged matcher needs for a full 16x16-block tile (0: use the buffer path).
+// LDS bytes the staged matcher needs for a full 16 x 8*WR block tile (0: does not fit).
 constexpr size_t kMatchLdsBudget = 80 * 1024;  // two 256-lane workgroups per CU
 
-size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride) {
+size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride, int wr = 2) {
     const long long rw = 15LL * block_step + 2LL * bm_range + 9;
-    const long long rh = 15LL * block_step + 2LL * bm_range + 8;
+    const long long rh = (8LL * wr - 1) * block_step + 2LL * bm_range + 8;
     const long long ls = (rw + 3 + 31) / 32 * 32;
     const size_t bytes = (size_t)(rh * ls * 4);
     if (lds_stride) *lds_stride = (int)ls;
@@ -1468,7 +1469,8 @@ template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
     c.no_self = 1;
     int ls = 0;
     const bool lds4 = (M == 0 || M == 1) && match_lds4_bytes(a, &ls) != 0;
-    if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls)) return false;
+    if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls, 2) && !match_lds_bytes(a.block_step, a.bm_range, &ls, 1))
+        return false;
     launch_match_m<M>(c, false, s);
     const dim3 grid((unsigned)(a.tiles_x * ((a.nby + 7) / 8)), 1, 1);
     hipLaunchKernelGGL((k_match_temporal<M>), grid, dim3(64), 0, s, a);
@@ -1491,11 +1493,17 @@ template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStrea
     }
     if (!temporal) {
         MatchArgs b = a;
-        const size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride);
+        b.tiles_x = (a.nbx + 15) / 16;
+        size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 2);
         if (lds) {
-            b.tiles_x = (a.nbx + 15) / 16;
             const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 15) / 16)), 1, 1);
-            hipLaunchKernelGGL((k_match_lds<M>), grid, dim3(256), lds, s, b);
+            hipLaunchKernelGGL((k_match_lds<M, 2>), grid, dim3(256), lds, s, b);
+            return;
+        }
+        lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 1);   // e.g. block_step 8
+        if (lds) {
+            const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 7) / 8)), 1, 1);
+            hipLaunchKernelGGL((k_match_lds<M, 1>), grid, dim3(128), lds, s, b);
             return;
         }
     }
