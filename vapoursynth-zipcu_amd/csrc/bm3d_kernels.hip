@@ -456,15 +456,19 @@ __device__ __forceinline__ void lds_step(float (&w)[9][8], const char *lds, cons
     if (row_ok && xt >= 0 && xt <= W - 8) insert8<false>(L, e, (unsigned)xt | ybits, 0);
 }
 
-// WR wave rows: a workgroup of 128 * WR lanes covers 16 x 8*WR blocks.
-template <int M, int WR = 2>
+// WR wave rows: a workgroup of 128 * WR lanes. SPLIT lanes share a block: each scans a
+// contiguous band of its search rows, and the later bands' sorted lists are inserted into the
+// earlier ones with the same strict-< rule, which reproduces one sequential scan exactly. A wave
+// covers 8 x (8 / SPLIT) blocks, the workgroup 16 x WR * (8 / SPLIT).
+template <int M, int WR = 2, int SPLIT = 1>
 __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, float *region) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int W = a.width, H = a.height, S = a.stride, bs = a.block_step, Rg = a.bm_range;
-    const int bx0 = tx * 16, by0 = ty * (8 * WR);
+    constexpr int TH = WR * 8 / SPLIT;   // tile height in blocks
+    const int bx0 = tx * 16, by0 = ty * TH;
     const int x_first = min(bx0 * bs, W - 8), x_last = min(min(bx0 + 15, a.nbx - 1) * bs, W - 8);
-    const int y_first = min(by0 * bs, H - 8), y_last = min(min(by0 + 8 * WR - 1, a.nby - 1) * bs, H - 8);
+    const int y_first = min(by0 * bs, H - 8), y_last = min(min(by0 + TH - 1, a.nby - 1) * bs, H - 8);
     const int rx0 = x_first - Rg, ry0 = y_first - Rg;
     const int RW = x_last - x_first + 2 * Rg + 9, RH = y_last - y_first + 2 * Rg + 8;
     const int LS = a.lds_stride;
@@ -476,7 +480,9 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
     }
     __syncthreads();
 
-    const int bx = bx0 + (wave & 1) * 8 + (lane & 7), by = by0 + (wave >> 1) * 8 + (lane >> 3);
+    constexpr int PL = 64 / SPLIT;      // lanes per band in a wave
+    const int part = lane / PL, li = lane % PL;
+    const int bx = bx0 + (wave & 1) * 8 + (li & 7), by = by0 + (wave >> 1) * (8 / SPLIT) + (li >> 3);
     const bool valid = bx < a.nbx && by < a.nby;
     const int x = min(min(bx, a.nbx - 1) * bs, W - 8);
     const int y = min(min(by, a.nby - 1) * bs, H - 8);
@@ -492,7 +498,9 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
     const char *lds = reinterpret_cast<const char *>(region);
     const int span = 2 * Rg;
     const int X0 = x - Rg - rx0;
-    for (int dy = -Rg; dy <= Rg; ++dy) {
+    const int nrows = 2 * Rg + 1;
+    const int dy_end = -Rg + (nrows * (part + 1)) / SPLIT;
+    for (int dy = -Rg + (nrows * part) / SPLIT; dy < dy_end; ++dy) {
         const int yy = y + dy;
         const bool row_ok = yy >= 0 && yy <= H - 8;
         const unsigned ybits = (unsigned)yy << 16;
@@ -533,6 +541,23 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
         }
     }
 
+    // merge the bands in scan order: (0 <- 1, 2 <- 3), then 0 <- 2
+#pragma unroll
+    for (int d = PL; d < 64; d *= 2) {
+        float oe[8];
+        unsigned op[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            oe[k] = __shfl_xor(L.er[k], d);
+            op[k] = (unsigned)__shfl_xor((int)L.pos[k], d);
+        }
+        if ((part & ((2 * d / PL) - 1)) == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) insert8<false>(L, oe[k], op[k], 0);
+        }
+    }
+    if (part != 0) return;
+
     const unsigned self = pack_xy(x, y);
     bool found = a.no_self != 0;   // a temporal stage does the self-inclusion after its merges
 #pragma unroll
@@ -552,10 +577,10 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
 }
 
-template <int M, int WR>
+template <int M, int WR, int SPLIT>
 __global__ __launch_bounds__(128 * WR) void k_match_lds(MatchArgs a) {
     extern __shared__ float region[];
-    match_lds_body<M, WR>(a, xcd_tile(blockIdx.x, gridDim.x), region);
+    match_lds_body<M, WR, SPLIT>(a, xcd_tile(blockIdx.x, gridDim.x), region);
 }
 
 // ---- shared-column matching for block_step 4 (ssd, sad) ---------------------------------------
@@ -1439,9 +1464,9 @@ Layout layout_of(const vsz_bm3d_kparams *p) {
 // LDS bytes the staged matcher needs for a full 16 x 8*WR block tile (0: does not fit).
 constexpr size_t kMatchLdsBudget = 80 * 1024;  // two 256-lane workgroups per CU
 
-size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride, int wr = 2) {
+size_t match_lds_bytes(int block_step, int bm_range, int *lds_stride, int wr = 2, int split = 1) {
     const long long rw = 15LL * block_step + 2LL * bm_range + 9;
-    const long long rh = (8LL * wr - 1) * block_step + 2LL * bm_range + 8;
+    const long long rh = (8LL * wr / split - 1) * block_step + 2LL * bm_range + 8;
     const long long ls = (rw + 3 + 31) / 32 * 32;
     const size_t bytes = (size_t)(rh * ls * 4);
     if (lds_stride) *lds_stride = (int)ls;
@@ -1469,7 +1494,7 @@ template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
     c.no_self = 1;
     int ls = 0;
     const bool lds4 = (M == 0 || M == 1) && match_lds4_bytes(a, &ls) != 0;
-    if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls, 2) && !match_lds_bytes(a.block_step, a.bm_range, &ls, 1))
+    if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls, 2, 4) && !match_lds_bytes(a.block_step, a.bm_range, &ls, 1))
         return false;
     launch_match_m<M>(c, false, s);
     const dim3 grid((unsigned)(a.tiles_x * ((a.nby + 7) / 8)), 1, 1);
@@ -1492,18 +1517,24 @@ template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStrea
         }
     }
     if (!temporal) {
+        // split each block's rows over 2 or 4 lanes when the plane has too few blocks to fill
+        // the chip (about 2048 waves)
+        const long long nb = (long long)a.nbx * a.nby;
+        const int split = nb >= 2048LL * 64 ? 1 : nb >= 1024LL * 64 ? 2 : 4;
         MatchArgs b = a;
         b.tiles_x = (a.nbx + 15) / 16;
-        size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 2);
+        size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 2, split);
         if (lds) {
-            const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 15) / 16)), 1, 1);
-            hipLaunchKernelGGL((k_match_lds<M, 2>), grid, dim3(256), lds, s, b);
+            const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 16 / split - 1) / (16 / split))), 1, 1);
+            if (split == 1) hipLaunchKernelGGL((k_match_lds<M, 2, 1>), grid, dim3(256), lds, s, b);
+            else if (split == 2) hipLaunchKernelGGL((k_match_lds<M, 2, 2>), grid, dim3(256), lds, s, b);
+            else hipLaunchKernelGGL((k_match_lds<M, 2, 4>), grid, dim3(256), lds, s, b);
             return;
         }
-        lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 1);   // e.g. block_step 8
+        lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 1, 1);   // e.g. block_step 8
         if (lds) {
             const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 7) / 8)), 1, 1);
-            hipLaunchKernelGGL((k_match_lds<M, 1>), grid, dim3(128), lds, s, b);
+            hipLaunchKernelGGL((k_match_lds<M, 1, 1>), grid, dim3(128), lds, s, b);
             return;
         }
     }
