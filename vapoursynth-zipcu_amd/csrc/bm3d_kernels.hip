@@ -813,12 +813,18 @@ __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
 // LDS-staged matcher (without self-inclusion): the predictive search in the other frames, the
 // merges and the self-inclusion, exactly as the single-kernel path above does after its spatial
 // scan (src/bm3d.cu:663-788).
+// Two lanes per block, one per search direction: the lane for dir = -1 merges its frames' entries
+// into the list as the single-lane order does; the lane for dir = +1 leaves its entries
+// (ps_num per frame, frames in order) in LDS, and its partner inserts them afterwards, which is
+// the same sequence of insertions. A 64-lane workgroup covers 8 x 4 blocks.
 template <int M>
 __global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
+    extern __shared__ int4 fwd[];               // [32 blocks][radius * ps_num] (x, y, z, error bits)
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
+    const int bl = lane & 31, dir = lane < 32 ? -1 : 1;
+    const int bx = tx * 8 + (bl & 7), by = ty * 4 + (bl >> 3);
     const bool valid = bx < a.nbx && by < a.nby;
     const int W = a.width, H = a.height, S = a.stride, Rt = a.radius;
     const int x = min(min(bx, a.nbx - 1) * a.block_step, W - 8);
@@ -842,28 +848,33 @@ __global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
         L.pos[k] = pack_xy(m.x, m.y);
         L.zs[k] = Rt;
     }
-    unsigned cpos[8];
+    unsigned lpos[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) cpos[k] = L.pos[k];
-    for (int dir = -1; dir <= 1; dir += 2) {
-        unsigned lpos[8];
+    for (int k = 0; k < 8; ++k) lpos[k] = L.pos[k];
+    int4 *mine = fwd + bl * (Rt * a.ps_num);
+    for (int t = 1; t <= Rt; ++t) {
+        const int tz = Rt + dir * t;
+        List8 F;
+        list_init<false>(F, tz);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) lpos[k] = cpos[k];
-        for (int t = 1; t <= Rt; ++t) {
-            const int tz = Rt + dir * t;
-            List8 F;
-            list_init<false>(F, tz);
+        for (int i = 0; i < 8; ++i)
+            if (i < a.ps_num)
+                scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
+                                      a.ps_range, tz, F);
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (i < a.ps_num)
-                    scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
-                                          a.ps_range, tz, F);
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-                if (i < a.ps_num) insert8<true>(L, F.er[i], F.pos[i], tz);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) lpos[k] = F.pos[k];
+        for (int i = 0; i < 8; ++i) {
+            if (i >= a.ps_num) continue;
+            if (dir < 0) insert8<true>(L, F.er[i], F.pos[i], tz);
+            else mine[(t - 1) * a.ps_num + i] = make_int4(pos_x(F.pos[i]), pos_y(F.pos[i]), tz, __float_as_int(F.er[i]));
         }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) lpos[k] = F.pos[k];
+    }
+    __syncthreads();
+    if (dir > 0 || !valid) return;
+    for (int e = 0; e < Rt * a.ps_num; ++e) {
+        const int4 m = mine[e];
+        insert8<true>(L, __int_as_float(m.w), pack_xy(m.x, m.y), m.z);
     }
 
     const unsigned self = pack_xy(x, y);
@@ -881,7 +892,6 @@ __global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
         L.pos[0] = self;
         L.zs[0] = Rt;
     }
-    if (!valid) return;
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), L.zs[k], __float_as_int(L.er[k]));
 }
@@ -1497,8 +1507,11 @@ template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
     if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls, 2, 4) && !match_lds_bytes(a.block_step, a.bm_range, &ls, 1))
         return false;
     launch_match_m<M>(c, false, s);
-    const dim3 grid((unsigned)(a.tiles_x * ((a.nby + 7) / 8)), 1, 1);
-    hipLaunchKernelGGL((k_match_temporal<M>), grid, dim3(64), 0, s, a);
+    MatchArgs t = a;
+    t.tiles_x = (a.nbx + 7) / 8;
+    const dim3 grid((unsigned)(t.tiles_x * ((a.nby + 3) / 4)), 1, 1);
+    const size_t lds = (size_t)32 * a.radius * a.ps_num * sizeof(int4);
+    hipLaunchKernelGGL((k_match_temporal<M>), grid, dim3(64), lds, s, t);
     return true;
 }
 
