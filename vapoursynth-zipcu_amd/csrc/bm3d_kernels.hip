@@ -343,6 +343,11 @@ template <int IMM> __device__ __forceinline__ float buf_load(const PlaneBuf &b, 
     return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(b.rsrc, voff + IMM, 0, 0));
 }
 
+// Per-lane byte offset plus a wave-uniform one (an SGPR operand: no address arithmetic per load).
+__device__ __forceinline__ float buf_load_s(const PlaneBuf &b, int voff, int soff) {
+    return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(b.rsrc, voff, soff, 0));
+}
+
 // Loads column (step U + J) of the 8 rows whose byte offsets (at the block's first column) are vo[r].
 template <int IMM> __device__ __forceinline__ void load_col(float (&col)[8], const PlaneBuf &b, const int (&vo)[8]) {
 #pragma unroll
@@ -1047,25 +1052,40 @@ __device__ __forceinline__ float bfly8(float x) {
     return x;
 }
 
-__device__ __forceinline__ long long to_fixed(float q, int e) {
-    float s = __builtin_ldexpf(q, 24 - e);
+// Fixed-point value of an accumulator term already scaled by 2^(24-e) (an integer, or beyond
+// the clamp): the 64-bit accumulator counts units of 2^(e-24).
+__device__ __forceinline__ long long fixed_from_scaled(float s) {
     s = fminf(fmaxf(s, -4.611686018427387904e18f), 4.611686018427387904e18f);
     return (long long)s;
 }
 
-// Loads the group (and, in the final pass, the basic-estimate group) in column layout.
+__device__ __forceinline__ long long to_fixed(float q, int e) {
+    return fixed_from_scaled(__builtin_ldexpf(q, 24 - e));
+}
+
+// The quantised term q = (w*v + E) - E with E = 2^e (src/bm3d.cu:884-885), computed directly
+// in accumulator units as (ws*v + 2^24) - 2^24 with ws = w * 2^(24-e). Scaling by a power of two
+// commutes with every rounding step here: all operands stay normal or the term rounds to zero
+// in both forms, so the result is exactly q * 2^(24-e).
+constexpr float kFixOne = 16777216.0f;  // 2^24 = E * 2^(24-e)
+__device__ __forceinline__ float scaled_term(float ws, float v) { return (ws * v + kFixOne) - kFixOne; }
+
+// Loads the group (and, in the final pass, the basic-estimate group) in column layout. The
+// [TW][H][S] plane stack is below 2 GiB, so every offset fits the 32-bit buffer addressing.
 template <bool FINAL>
 __device__ __forceinline__ void gather_group(const FilterArgs &a, int p, const int *mx, const int *my, const int *mz,
                                              int c, float *v, float *r) {
-    const float *pref = a.clip0 + (long long)p * a.plane_elems;
-    const float *pnoisy = FINAL ? a.clip1 + (long long)p * a.plane_elems : pref;
+    const unsigned bytes = (unsigned)(a.plane_elems * 4);
+    const PlaneBuf bref = make_plane_buf(a.clip0 + (long long)p * a.plane_elems, bytes);
+    const PlaneBuf bnoisy = FINAL ? make_plane_buf(a.clip1 + (long long)p * a.plane_elems, bytes) : bref;
+    const int row_bytes = a.stride * 4;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const long long off = (long long)mz[i] * a.ts + (long long)my[i] * a.stride + mx[i] + c;
+        const int off = (int)(mz[i] * a.ts + my[i] * a.stride + mx[i] + c) * 4;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            v[i * 8 + j] = pnoisy[off + (long long)j * a.stride];
-            if constexpr (FINAL) r[i * 8 + j] = pref[off + (long long)j * a.stride];
+            v[i * 8 + j] = buf_load_s(bnoisy, off, j * row_bytes);
+            if constexpr (FINAL) r[i * 8 + j] = buf_load_s(bref, off, j * row_bytes);
         }
     }
 }
@@ -1093,13 +1113,17 @@ __device__ __forceinline__ float filter_group(float *v, float *r, float sigma, f
 #pragma unroll
         for (int f = 0; f < 64; ++f) k = fmaf_(r[f], r[f], k);
     } else {
+        // k counts the kept coefficients; as a float sum of ones it is exact, so an integer count
+        // gives the same value
+        int kept = 0;
 #pragma unroll
         for (int f = 0; f < 64; ++f) {
             const float thr = (f == 0 && l8 == 0) ? 0.0f : sigma;
             const bool keep = fabsf(v[f]) >= thr;
-            k = k + (keep ? 1.0f : 0.0f);
+            kept += keep ? 1 : 0;
             v[f] = keep ? v[f] * (1.0f / 4096.0f) : 0.0f;
         }
+        k = (float)kept;
     }
     const float w = __fdiv_rn(1.0f, bfly8(k));
     inverse3d<T2, T1>(v, buf, l8);
@@ -1154,15 +1178,15 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
         transpose_ranks(v, buf, c);
         if (!valid) continue;
         const long long pb = (long long)p * a.acc_plane_elems;
+        const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {  // column layout: this lane adds column (lane & 7) of patch i
             const long long base = pb + (long long)mz[i] * 2 * a.ts + (long long)my[i] * a.stride + mx[i] + c;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const float q = (w * v[i * 8 + j] + E) - E;
                 if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + base + (long long)j * a.stride,
-                                               (unsigned long long)to_fixed(q, a.extractor_exp));
-                else atomicAdd((float *)a.acc + base + (long long)j * a.stride, q);
+                                               (unsigned long long)fixed_from_scaled(scaled_term(ws, v[i * 8 + j])));
+                else atomicAdd((float *)a.acc + base + (long long)j * a.stride, (w * v[i * 8 + j] + E) - E);
             }
         }
         if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
@@ -1214,31 +1238,35 @@ __device__ __forceinline__ void filter_tile_body(const FilterArgs &a, int tile_x
             gather_group<FINAL>(a, p, mx, my, mz, c, v, r);
             const float w = filter_group<FINAL, T2, T1>(v, r, a.sigma[p], buf, c);
             if (!valid) continue;
+            const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
 #pragma unroll
             for (int i = 0; i < 8; ++i) {  // row layout: this lane adds row (lane & 7) of patch i
                 const bool inside = mx[i] >= rx0 && mx[i] + 8 <= rx1 && my[i] >= ry0 && my[i] + 8 <= ry1;
                 const int cell0 = (my[i] + c - ry0) * rw + mx[i] - rx0;
                 const long long gbase = pb + (long long)(my[i] + c) * a.stride + mx[i];
                 if constexpr (EXACT) {
-                    // q * 2^(24-e) is an integer; below the guard it converts exactly to int32.
-                    // Cells take 0 for contributions that must go to the device accumulator.
-                    const int cbase = inside ? cell0 : 0;
-                    bool spill = false;
+                    // Terms in accumulator units are integers; below the guard they convert
+                    // exactly to int32. The common case tests the row's largest magnitude once
+                    // (a NaN-propagating maximum, so a NaN term takes the per-term path).
                     float sq[8];
+                    float big = 0.0f;
 #pragma unroll
                     for (int cc = 0; cc < 8; ++cc) {
-                        sq[cc] = __builtin_ldexpf((w * v[i * 8 + cc] + E) - E, 24 - a.extractor_exp);
-                        const bool ok = inside && fabsf(sq[cc]) < guard_f;
-                        spill |= !ok;
-                        atomicAdd(cell_i + cbase + cc, ok ? (int)sq[cc] : 0);
+                        sq[cc] = scaled_term(ws, v[i * 8 + cc]);
+                        big = __builtin_elementwise_maximum(big, fabsf(sq[cc]));
                     }
-                    if (spill) {
+                    if (inside && big < guard_f) {
 #pragma unroll
-                        for (int cc = 0; cc < 8; ++cc)
-                            if (!(inside && fabsf(sq[cc]) < guard_f))
+                        for (int cc = 0; cc < 8; ++cc) atomicAdd(cell_i + cell0 + cc, (int)sq[cc]);
+                    } else {
+#pragma unroll
+                        for (int cc = 0; cc < 8; ++cc) {
+                            if (inside && fabsf(sq[cc]) < guard_f)
+                                atomicAdd(cell_i + cell0 + cc, (int)sq[cc]);
+                            else
                                 atomicAdd((unsigned long long *)a.acc + gbase + cc,
-                                          (unsigned long long)to_fixed(__builtin_ldexpf(sq[cc], a.extractor_exp - 24),
-                                                                       a.extractor_exp));
+                                          (unsigned long long)fixed_from_scaled(sq[cc]));
+                        }
                     }
                 } else {
 #pragma unroll
