@@ -628,9 +628,10 @@ __device__ __forceinline__ void lds4_step(float (&w)[5][8], const lds_cptr (&pr)
         }
         own[c] = e0 + e1;
     }
-    const float col[8] = {dpp_from_left(own[0]), dpp_from_left(own[1]), dpp_from_left(own[2]), dpp_from_left(own[3]),
-                          own[0], own[1], own[2], own[3]};
-    const float e = tree8(col);
+    // tree8 of {left neighbour's own[0..3], own[0..3]}: the shift is a lane permutation, so the
+    // left half ((c0 + c1) + (c2 + c3)) is the shifted right half, summed once
+    const float half_sum = (own[0] + own[1]) + (own[2] + own[3]);
+    const float e = dpp_from_left(half_sum) + half_sum;
     if (ok) insert8<false>(L, e, pos, 0);
 }
 
@@ -688,15 +689,17 @@ __device__ __forceinline__ void match_lds4_body(const MatchArgs &a, int tile, fl
         const int x0 = x - Rg;
         for (int tb = 0; tb <= span; tb += 5) {
             const int xt = x0 + tb;
-            lds4_step<M, 0>(w, pr, ref, row_ok && xt >= 0 && xt <= W - 8, (unsigned)xt | ybits, L);
+            // one unsigned compare per candidate: a negative x wraps above W - 8
+            const unsigned xmax = (unsigned)(W - 8);
+            lds4_step<M, 0>(w, pr, ref, row_ok && (unsigned)xt <= xmax, (unsigned)xt | ybits, L);
             if (tb + 1 > span) break;
-            lds4_step<M, 1>(w, pr, ref, row_ok && xt + 1 >= 0 && xt + 1 <= W - 8, (unsigned)(xt + 1) | ybits, L);
+            lds4_step<M, 1>(w, pr, ref, row_ok && (unsigned)(xt + 1) <= xmax, (unsigned)(xt + 1) | ybits, L);
             if (tb + 2 > span) break;
-            lds4_step<M, 2>(w, pr, ref, row_ok && xt + 2 >= 0 && xt + 2 <= W - 8, (unsigned)(xt + 2) | ybits, L);
+            lds4_step<M, 2>(w, pr, ref, row_ok && (unsigned)(xt + 2) <= xmax, (unsigned)(xt + 2) | ybits, L);
             if (tb + 3 > span) break;
-            lds4_step<M, 3>(w, pr, ref, row_ok && xt + 3 >= 0 && xt + 3 <= W - 8, (unsigned)(xt + 3) | ybits, L);
+            lds4_step<M, 3>(w, pr, ref, row_ok && (unsigned)(xt + 3) <= xmax, (unsigned)(xt + 3) | ybits, L);
             if (tb + 4 > span) break;
-            lds4_step<M, 4>(w, pr, ref, row_ok && xt + 4 >= 0 && xt + 4 <= W - 8, (unsigned)(xt + 4) | ybits, L);
+            lds4_step<M, 4>(w, pr, ref, row_ok && (unsigned)(xt + 4) <= xmax, (unsigned)(xt + 4) | ybits, L);
 #pragma unroll
             for (int r = 0; r < 8; ++r) pr[r] += 20;
         }
