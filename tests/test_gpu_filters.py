@@ -140,3 +140,25 @@ def test_concurrent_get_frame(torch_gpu):
         t.join()
     for n in range(nf):
         assert np.array_equal(got[n], serial[n])
+
+
+def test_zero_copy_output_matches_copy_path(torch_gpu):
+    """num_streams >= 2 writes the output frame from the kernels straight into mapped pinned
+    memory (src/bm3d.zig:1203-1206); num_streams = 1 copies it device-to-host. Same bits, for a
+    radius-0 pass with an unprocessed (passthrough) plane and for the fused temporal chain."""
+    W, H, nf = 64, 48, 4
+    clean, nz = synth.moving_sequence(W, H, nf, 8.0, seed=21)
+    yuv = Clip.from_frames([[f, f[::-1].copy(), f[:, ::-1].copy()] for f in nz], YUV444PS)
+    gray = Clip.from_frames(list(nz), GRAYS)
+    cases = [(core.vszipcu.BM3D, yuv, dict(sigma=[8.0, 0.0, 6.0], block_step=4, bm_range=6, extractor_exp=3)),
+             (core.vszipcu.BM3Dv2, gray, dict(sigma=8.0, block_step=4, bm_range=6, radius=1, fast_fused=1,
+                                              extractor_exp=3))]
+    for fn, clip, kw in cases:
+        dma = fn(clip, num_streams=1, **kw)
+        zc = fn(clip, num_streams=4, **kw)
+        for n in range(nf):
+            a, b = dma.get_frame(n).planes, zc.get_frame(n).planes
+            for p in range(len(a)):
+                assert np.array_equal(a[p], b[p]), (fn, n, p)
+        if fn is core.vszipcu.BM3D:
+            assert np.array_equal(zc.get_frame(1).planes[1], nz[1][::-1])  # passthrough plane

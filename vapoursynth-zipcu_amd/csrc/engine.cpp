@@ -257,6 +257,7 @@ struct StreamSlot {
     float *d_out = nullptr;   // dst (normal output) or res (stacked output)
     char *d_work = nullptr;   // kernel workspaces of all entries
     float *h_stage = nullptr; // pinned: upload region + download region
+    float *d_down = nullptr;  // zero-copy output: device address of the download region
 
     void destroy() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -392,6 +393,11 @@ class Bm3d final : public vsz_filter {
     }
 
     bool normal_out() const { return radius == 0 || fused; }
+    // Zero-copy output (src/bm3d.zig:1203-1206): with two or more streams the output kernels
+    // write the normal (dst) frame straight into mapped pinned memory, so no D2H copy runs.
+    bool zc_dst() const { return normal_out() && num_streams >= 2; }
+    size_t down_off() const { return std::max(sum_src, cache_elems); }
+    float *out_base(StreamSlot &s) const { return zc_dst() ? s.d_down : s.d_out; }
     const char *frame_error() const override { return v2_error.c_str(); }
     // src/bm3d.zig:643-651: n-r..n+r (n-2r..n+2r when fused) of clip and ref
     void frame_range(int which, int n, int &lo, int &hi) const override {
@@ -635,9 +641,14 @@ int Bm3d::init_device(char *err, size_t errlen) {
         auto s = std::make_unique<StreamSlot>();
         bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess;
         ok = ok && hipMalloc(&s->d_src, std::max<size_t>(sum_src, 64) * 4) == hipSuccess;
-        ok = ok && hipMalloc(&s->d_out, std::max<size_t>(out_elems, 64) * 4) == hipSuccess;
+        ok = ok && hipMalloc(&s->d_out, std::max<size_t>(zc_dst() ? 0 : out_elems, 64) * 4) == hipSuccess;
         ok = ok && hipMalloc(&s->d_work, std::max<size_t>(sum_work, 256)) == hipSuccess;
-        ok = ok && hipHostMalloc(&s->h_stage, (std::max(sum_src, cache_elems) + out_elems + 64) * 4) == hipSuccess;
+        ok = ok && hipHostMalloc(&s->h_stage, (down_off() + out_elems + 64) * 4, hipHostMallocMapped) == hipSuccess;
+        if (ok && zc_dst()) {
+            void *dp = nullptr;
+            ok = hipHostGetDevicePointer(&dp, s->h_stage, 0) == hipSuccess && dp != nullptr;
+            if (ok) s->d_down = static_cast<float *>(dp) + down_off();
+        }
         if (!ok) {
             (void)hipGetLastError();
             s->destroy();
@@ -762,8 +773,9 @@ int Bm3d::launch_entries(StreamSlot &s, float *out_base, int out_kind) {
 // ---- download (src/bm3d.zig:427-478) ---------------------------------------------------------
 int Bm3d::download(StreamSlot &s, const float *d_base, bool stacked, void *const dst[3], const ptrdiff_t dst_stride[3]) {
     const size_t n = stacked ? sum_res : sum_dst;
-    float *h = s.h_stage + std::max(sum_src, cache_elems);
-    HIPCHK(hipMemcpyAsync(h, d_base, n * 4, hipMemcpyDeviceToHost, s.stream));
+    float *h = s.h_stage + down_off();
+    if (d_base != s.d_down)  // zero-copy: the kernels already wrote the download region
+        HIPCHK(hipMemcpyAsync(h, d_base, n * 4, hipMemcpyDeviceToHost, s.stream));
     HIPCHK(hipStreamSynchronize(s.stream));
     for (auto &e : entries)
         for (int i = 0; i < e.n_planes; ++i) {
@@ -794,8 +806,9 @@ int Bm3d::process_normal(StreamSlot &s, int n, vsz_get_plane_fn get, void *user,
         for (int t = 0; t < tw; ++t) slot_of_t[t] = t;
         rc = gather_from_cache(*this, s, cache, held, tw, slot_of_t);
     }
-    if (rc == VSZ_OK) rc = launch_entries(s, s.d_out, radius == 0 ? 0 : 1);
-    if (rc == VSZ_OK) rc = download(s, s.d_out, radius > 0, dst, dst_stride);
+    float *out = radius == 0 ? out_base(s) : s.d_out;
+    if (rc == VSZ_OK) rc = launch_entries(s, out, radius == 0 ? 0 : 1);
+    if (rc == VSZ_OK) rc = download(s, out, radius > 0, dst, dst_stride);
     if (rc != VSZ_OK) (void)hipStreamSynchronize(s.stream);  // drain before releasing slots
     if (!held.empty()) {
         for (size_t k = 0; k < held.size(); ++k)
@@ -854,12 +867,12 @@ int Bm3d::process_fused(StreamSlot &s, int n, vsz_get_plane_fn get, void *user, 
                 for (int i = 0; i < tw; ++i)
                     slots[i] = static_cast<const float *>(acc.slots[aidx[i]].buf) + e.res_off +
                                size_t(pi) * tw * 2 * e.plane_extent;
-                rc = vsz_vaggregate_run(e.w, e.h, e.S, r, n, nf, slots, s.d_out + e.dst_off + size_t(pi) * e.plane_extent,
-                                        s.stream);
+                rc = vsz_vaggregate_run(e.w, e.h, e.S, r, n, nf, slots,
+                                        out_base(s) + e.dst_off + size_t(pi) * e.plane_extent, s.stream);
             }
         }
     }
-    if (rc == VSZ_OK) rc = download(s, s.d_out, false, dst, dst_stride);
+    if (rc == VSZ_OK) rc = download(s, out_base(s), false, dst, dst_stride);
     if (rc != VSZ_OK) (void)hipStreamSynchronize(s.stream);
     if (acc_held) {
         for (int i = 0; i < tw; ++i)
