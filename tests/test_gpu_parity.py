@@ -167,12 +167,13 @@ def test_lds_matcher_short_tiles(orc, torch_gpu, W, H, bs, rng, metric):
     assert_matches_equal(m.cpu().numpy(), om, oe)
 
 
-@pytest.mark.parametrize("bs,rng", [(8, 9), (4, 16)])
-def test_temporal_split_matcher(orc, torch_gpu, bs, rng):
-    """radius > 0: centre-frame search on an LDS-staged matcher, then the predictive search."""
-    print(f"temporal split: bs={bs} R={rng}", flush=True)
+@pytest.mark.parametrize("bs,rng,ps_num,ps_range", [(8, 9, 2, 3), (4, 16, 2, 3), (8, 9, 2, 4), (4, 16, 3, 4)])
+def test_temporal_split_matcher(orc, torch_gpu, bs, rng, ps_num, ps_range):
+    """radius > 0: centre-frame search on an LDS-staged matcher, then the predictive search
+    (range 4 takes the register-band scanner, whose windows here often cross the frame edges)."""
+    print(f"temporal split: bs={bs} R={rng} ps_num={ps_num} ps_range={ps_range}", flush=True)
     _, seq = synth.moving_sequence(72, 64, 5, 8.0, seed=41)
     src = np.stack(seq)[None, None]                            # [1, 1, 5, H, W]
-    kp, op = kp_and_op(orc, 72, 64, bs=bs, rng=rng, radius=2, ps_num=2, ps_range=3,
+    kp, op = kp_and_op(orc, 72, 64, bs=bs, rng=rng, radius=2, ps_num=ps_num, ps_range=ps_range,
                        sigma=device.scaled_sigma([8.0] * 3, False))
     _check(orc, torch_gpu, kp, op, src)

@@ -429,6 +429,70 @@ __device__ __forceinline__ void scan_window(const PlaneBuf &b, long long frame_o
     }
 }
 
+// Image row y, columns x0 .. x0+15 of one frame. Inside the row: four 16-byte vector loads;
+// otherwise per-column loads clamped to the row (only candidates that are never inserted read
+// the clamped columns). Rows outside the frame read other frames' rows or, beyond the stack,
+// zeros; they too belong to candidates that are never inserted.
+__device__ __forceinline__ void load_band_row(float (&row)[16], const PlaneBuf &b, int row_byte, int x0, int W) {
+    if (x0 >= 0 && x0 + 16 <= W) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 v = __builtin_bit_cast(int4, __builtin_amdgcn_raw_buffer_load_b128(b.rsrc, row_byte + 4 * x0 + 16 * q, 0, 0));
+            row[4 * q + 0] = __int_as_float(v.x);
+            row[4 * q + 1] = __int_as_float(v.y);
+            row[4 * q + 2] = __int_as_float(v.z);
+            row[4 * q + 3] = __int_as_float(v.w);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) row[j] = buf_load<0>(b, row_byte + 4 * min(max(x0 + j, 0), W - 1));
+    }
+}
+
+// Predictive-search window of range 4 (9 x 9 candidates, src/bm3d.cu:690-727) in the reference's
+// row-major order, from a band of 8 image rows x 16 columns held in registers: each search row
+// loads only the image row that enters the band, one search row ahead. Same distances and
+// insertion order as scan_window.
+template <int M>
+__device__ __forceinline__ void scan_band4(const PlaneBuf &b, long long frame_off, int S, int W, int H,
+                                           const RefPatch &R, int cx, int cy, int z, List8 &F) {
+    constexpr int kRng = 4;
+    const int x0 = cx - kRng, y0 = cy - kRng;
+    const int row_bytes = S * 4, base = (int)(frame_off * 4) + y0 * row_bytes;
+    float band[8][16], next[16];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) load_band_row(band[r], b, base + r * row_bytes, x0, W);
+    for (int dy = 0; dy <= 2 * kRng; ++dy) {
+        // the row entering the band at the next search row, loaded a row ahead
+        if (dy < 2 * kRng) load_band_row(next, b, base + (dy + 8) * row_bytes, x0, W);
+        const int yy = y0 + dy;
+        const bool row_ok = yy >= 0 && yy <= H - 8;
+        const unsigned ybits = (unsigned)yy << 16;
+#pragma unroll
+        for (int t = 0; t <= 2 * kRng; ++t) {
+            float w[8][8];  // w[column][row] of candidate t
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) w[c][r] = band[r][t + c];
+            float w9[9][8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) w9[c][r] = w[c][r];
+            const float e = win_dist<M, 0>(R, w9);
+            const int xt = x0 + t;
+            if (row_ok && xt >= 0 && xt <= W - 8) insert8<false>(F, e, (unsigned)xt | ybits, z);
+        }
+#pragma unroll
+        for (int r = 0; r < 7; ++r)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) band[r][j] = band[r + 1][j];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) band[7][j] = next[j];
+    }
+}
+
 // ---- LDS-staged spatial matching -------------------------------------------------------------
 // A 256-lane workgroup owns 16x16 reference blocks (2x2 waves of 8x8). It stages the union of their
 // search windows, [x_first-R, x_last+R+9) x [y_first-R, y_last+R+8) of the plane, in LDS once
@@ -780,11 +844,16 @@ __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
                 const int tz = Rt + dir * t;
                 List8 F;
                 list_init<false>(F, tz);
+                if (a.ps_range == 4) {
+                    for (int i = 0; i < a.ps_num; ++i)
+                        scan_band4<M>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]), tz, F);
+                } else {
 #pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    if (i < a.ps_num)
-                        scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
-                                              a.ps_range, tz, F);
+                    for (int i = 0; i < 8; ++i)
+                        if (i < a.ps_num)
+                            scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]),
+                                                  pos_y(lpos[i]), a.ps_range, tz, F);
+                }
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
                     if (i < a.ps_num) insert8<true>(L, F.er[i], F.pos[i], tz);
@@ -864,11 +933,16 @@ __global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
         const int tz = Rt + dir * t;
         List8 F;
         list_init<false>(F, tz);
+        if (a.ps_range == 4) {
+            for (int i = 0; i < a.ps_num; ++i)
+                scan_band4<M>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]), tz, F);
+        } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (i < a.ps_num)
-                scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
-                                      a.ps_range, tz, F);
+            for (int i = 0; i < 8; ++i)
+                if (i < a.ps_num)
+                    scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
+                                          a.ps_range, tz, F);
+        }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             if (i >= a.ps_num) continue;
