@@ -191,6 +191,126 @@ def bench_batch(args):
                                  "setup_s": round(t_setup, 2), "autotuned_tiles": tiles}}), flush=True)
 
 
+def _fused_chain(kp, frames, ref, work, acc, out, radius, stream):
+    """BM3Dv2 fused chain on device-resident frames [F, H, W]: per centre frame m, the window
+    clamp(m-r .. m+r) is gathered device-to-device (of `ref` too in a final pass) and one pass
+    writes that centre's [TW][2][H][W] accumulator stack; then VAggregate combines the 2r+1
+    stacks around each output frame (src/bm3d.zig:481-633, src/bm3d_vagg.cu)."""
+    import torch
+    from vszipcu import device
+    nf, TW = frames.shape[0], 2 * radius + 1
+    win0 = torch.empty((1, TW) + tuple(frames.shape[1:]), dtype=torch.float32, device="cuda")
+    win1 = torch.empty_like(win0) if ref is not None else None
+    for m in range(nf):
+        idx = torch.tensor([min(max(m - radius + t, 0), nf - 1) for t in range(TW)], device="cuda")
+        if ref is not None:   # final pass: clip 0 = ref (basic estimate), clip 1 = noisy
+            torch.index_select(ref, 0, idx, out=win0[0])
+            torch.index_select(frames, 0, idx, out=win1[0])
+        else:
+            torch.index_select(frames, 0, idx, out=win0[0])
+        device.run(kp, win0, win1, work, acc[m], 1, stream)
+    for n in range(nf):
+        slots = [acc[min(max(n - radius + i, 0), nf - 1)][0] for i in range(TW)]
+        device.vaggregate(frames.shape[2], radius, n, nf, slots, out[n], stream)
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_temporal(args):
+    """16 frames of 1920x1080 Gray with motion, sigma 8: BM3Dv2 basic pass at radius 2 (block_step
+    8, bm_range 9, ps 2/4, extractor 3) through the fused on-device accumulator chain, as plain FP32
+    and with split-BF16 DCT stages. One step = all 16 output frames."""
+    import torch
+    from vszipcu import device, synth
+    nf, r, sig = 16, 2, 8.0
+    clean, nz = synth.moving_sequence(W, H, nf, sig, seed=7)
+    frames = torch.from_numpy(nz).cuda()
+    kw = dict(radius=r, block_step=8, bm_range=9, ps_num=2, ps_range=4, extractor_exp=3)
+    acc = torch.empty((nf, 1, 2 * r + 1, 2, H, W), dtype=torch.float32, device="cuda")
+    out = torch.empty((nf, H, W), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, prec in (("plain_fp32", "fp32"), ("split_bf16_dct", "split-bf16")):
+        kp = device.kparams(W, H, sigma=device.scaled_sigma([sig] * 3, False), precision=prec, **kw)
+        work = torch.zeros(device.workspace_size(kp), dtype=torch.uint8, device="cuda")
+        dt = _timed(lambda: _fused_chain(kp, frames, None, work, acc, out, r, stream), args.steps, args.warmup)
+        got = out.cpu().numpy()
+        psnr = float(np.mean([synth.psnr(got[i], clean[i]) for i in range(nf)]))
+        res[name] = {"value": round(nf * W * H / dt / 1e6, 3), "ms_per_step": round(dt * 1e3, 3),
+                     "psnr_db": round(psnr, 4)}
+        print(f"bench: temporal {name} {dt * 1e3:.2f} ms / 16 frames, {nf * W * H / dt / 1e6:.1f} Mpix/s, "
+              f"PSNR {psnr:.3f} dB", file=sys.stderr, flush=True)
+    res["split_bf16_dct"]["psnr_delta_db"] = round(res["split_bf16_dct"]["psnr_db"] - res["plain_fp32"]["psnr_db"], 4)
+    print(json.dumps({"metric": "BM3Dv2 temporal (radius 2) 1080p Gray Mpix/s", "value": res["plain_fp32"]["value"],
+                      "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": res["plain_fp32"]["ms_per_step"], "higher_is_better": True, "dtype": "f32",
+                      "data": "synthetic",
+                      "config": {"workload": "16 frames 1920x1080 Gray with motion (seed 7), sigma 8, basic pass, "
+                                             "radius 2, bs 8, R 9, ps 2/4, extractor 3, fused accumulators + "
+                                             "VAggregate, noisy input " f"PSNR {np.mean([synth.psnr(nz[i], clean[i]) for i in range(nf)]):.2f} dB"},
+                      "forms": res}), flush=True)
+
+
+def bench_learned4k(args):
+    """8 frames of 3840x2160 Gray with motion, sigma 10: the Wiener pass at radius 1 (fused chain,
+    block_step 8, bm_range 9, extractor 3) with its reference from (a) the plain basic pass at
+    radius 1 or (b) the learned CNN basic estimate (BF16 MFMA, one call per frame). One step = all
+    8 output frames of both passes."""
+    import torch
+    from vszipcu import cnn, device, synth
+    W4, H4, nf, r, sig = 3840, 2160, 8, 1, 10.0
+    clean, nz = synth.moving_sequence(W4, H4, nf, sig, seed=8)
+    frames = torch.from_numpy(nz).cuda()
+    kw = dict(radius=r, block_step=8, bm_range=9, ps_num=2, ps_range=4, extractor_exp=3)
+    kb = device.kparams(W4, H4, sigma=device.scaled_sigma([sig] * 3, False), **kw)
+    kf = device.kparams(W4, H4, sigma=device.scaled_sigma([sig] * 3, True), final=True, **kw)
+    work = torch.zeros(max(device.workspace_size(kb), device.workspace_size(kf)), dtype=torch.uint8, device="cuda")
+    acc = torch.empty((nf, 1, 2 * r + 1, 2, H4, W4), dtype=torch.float32, device="cuda")
+    basic = torch.empty((nf, H4, W4), dtype=torch.float32, device="cuda")
+    out = torch.empty_like(basic)
+    stream = torch.cuda.current_stream()
+    model = cnn.Model.load()
+    cwork = torch.empty(_lib_cnn_ws(1, H4, W4), dtype=torch.uint8, device="cuda")
+
+    def plain():
+        _fused_chain(kb, frames, None, work, acc, basic, r, stream)
+        _fused_chain(kf, frames, basic, work, acc, out, r, stream)
+
+    def learned():
+        for i in range(nf):
+            model.run(frames[i], sig, out=basic[i], work=cwork, stream=stream)
+        _fused_chain(kf, frames, basic, work, acc, out, r, stream)
+
+    res = {}
+    for name, fn in (("plain", plain), ("learned_cnn_basic", learned)):
+        dt = _timed(fn, args.steps, args.warmup)
+        got = out.cpu().numpy()
+        psnr = float(np.mean([synth.psnr(got[i], clean[i]) for i in range(nf)]))
+        res[name] = {"value": round(nf * W4 * H4 / dt / 1e6, 3), "ms_per_step": round(dt * 1e3, 3),
+                     "psnr_db": round(psnr, 4)}
+        print(f"bench: 4K {name} {dt * 1e3:.2f} ms / {nf} frames, {nf * W4 * H4 / dt / 1e6:.1f} Mpix/s, "
+              f"PSNR {psnr:.3f} dB", file=sys.stderr, flush=True)
+    res["learned_cnn_basic"]["psnr_delta_db"] = round(res["learned_cnn_basic"]["psnr_db"] - res["plain"]["psnr_db"], 4)
+    print(json.dumps({"metric": "BM3Dv2 4K Gray Wiener radius 1 Mpix/s (plain vs learned basic pass)",
+                      "value": res["plain"]["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": res["plain"]["ms_per_step"], "higher_is_better": True,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "8 frames 3840x2160 Gray with motion (seed 8), sigma 10, basic + Wiener "
+                                             "at radius 1, bs 8, R 9, ps 2/4, extractor 3, fused accumulators + "
+                                             "VAggregate; learned form: CNN basic estimate per frame"},
+                      "forms": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,8 +319,10 @@ def main():
     ap.add_argument("--cpu-strip-rows", type=int, default=136, help="rows of the single-thread CPU sample")
     ap.add_argument("--no-end-to-end", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["frame1080p", "batch64"], default="frame1080p",
-                    help="batch64: 64 uneven Gray planes in one batched plan (not the headline line)")
+    ap.add_argument("--workload", choices=["frame1080p", "batch64", "temporal16", "learned4k"], default="frame1080p",
+                    help="batch64: 64 uneven Gray planes in one batched plan; temporal16: 16 frames 1080p Gray at "
+                         "radius 2 through the fused chain; learned4k: 4K Gray Wiener at radius 1 after the plain or "
+                         "the learned basic pass (none of them is the headline line)")
     ap.add_argument("--profile-target", choices=["step", "match", "copy"], default="step",
                     help="match: only the basic-pass matching of the bench frame, --steps launches (for "
                          "rocprofv3 --pmc runs); copy: a 256 MiB device copy, 3 times (counter calibration)")
@@ -218,6 +340,10 @@ def main():
         return profile_target(args)
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "temporal16":
+        return bench_temporal(args)
+    if args.workload == "learned4k":
+        return bench_learned4k(args)
     print(f"bench: workload 1920x1080 YUV444 two-pass BM3D, steps={args.steps} warmup={args.warmup}", file=sys.stderr,
           flush=True)
     clean, nz = synth.yuv444_frame(W, H, SIGMA, seed=2)
