@@ -451,20 +451,19 @@ __device__ __forceinline__ void load_band_row(float (&row)[16], const PlaneBuf &
 
 // Predictive-search window of range 4 (9 x 9 candidates, src/bm3d.cu:690-727) in the reference's
 // row-major order, from a band of 8 image rows x 16 columns held in registers: each search row
-// loads only the image row that enters the band, one search row ahead. Same distances and
-// insertion order as scan_window.
+// loads only the image row that enters the band. Same distances and insertion order as
+// scan_window.
 template <int M>
 __device__ __forceinline__ void scan_band4(const PlaneBuf &b, long long frame_off, int S, int W, int H,
                                            const RefPatch &R, int cx, int cy, int z, List8 &F) {
     constexpr int kRng = 4;
     const int x0 = cx - kRng, y0 = cy - kRng;
     const int row_bytes = S * 4, base = (int)(frame_off * 4) + y0 * row_bytes;
-    float band[8][16], next[16];
+    float band[8][16];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) load_band_row(band[r], b, base + r * row_bytes, x0, W);
+    for (int r = 0; r < 7; ++r) load_band_row(band[r], b, base + r * row_bytes, x0, W);
     for (int dy = 0; dy <= 2 * kRng; ++dy) {
-        // the row entering the band at the next search row, loaded a row ahead
-        if (dy < 2 * kRng) load_band_row(next, b, base + (dy + 8) * row_bytes, x0, W);
+        load_band_row(band[7], b, base + (dy + 7) * row_bytes, x0, W);
         const int yy = y0 + dy;
         const bool row_ok = yy >= 0 && yy <= H - 8;
         const unsigned ybits = (unsigned)yy << 16;
@@ -488,8 +487,6 @@ __device__ __forceinline__ void scan_band4(const PlaneBuf &b, long long frame_of
         for (int r = 0; r < 7; ++r)
 #pragma unroll
             for (int j = 0; j < 16; ++j) band[r][j] = band[r + 1][j];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) band[7][j] = next[j];
     }
 }
 
@@ -886,22 +883,25 @@ __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
         out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), TEMPORAL ? L.zs[k] : 0, __float_as_int(L.er[k]));
 }
 
-// Temporal stage of a radius > 0 pass whose centre-frame spatial lists were written by an
-// LDS-staged matcher (without self-inclusion): the predictive search in the other frames, the
-// merges and the self-inclusion, exactly as the single-kernel path above does after its spatial
-// scan (src/bm3d.cu:663-788).
-// Two lanes per block, one per search direction: the lane for dir = -1 merges its frames' entries
-// into the list as the single-lane order does; the lane for dir = +1 leaves its entries
-// (ps_num per frame, frames in order) in LDS, and its partner inserts them afterwards, which is
-// the same sequence of insertions. A 64-lane workgroup covers 8 x 4 blocks.
-template <int M>
-__global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
-    extern __shared__ int4 fwd[];               // [32 blocks][radius * ps_num] (x, y, z, error bits)
+// Temporal matching after the centre frame's spatial search: reads the centre lists, runs the
+// predictive search in the other frames (src/bm3d.cu:663-756) and the merges and the
+// self-inclusion in the reference's order. PW lanes per block and direction (PW = 2 when ps_num
+// is 2: one window each; PW = 1 scans the windows one after another). The window lanes' lists of
+// a frame are merged in window order (the later list's entries inserted into the earlier one with
+// the same strict-< rule, which equals one sequential scan). Each direction stores the first
+// ps_num entries of every frame's list (frames in order) in LDS, and after the scan the block's
+// first lane inserts the backward entries, then the forward ones, into the centre list: the same
+// sequence of insertions as one sequential search, without keeping the centre list in registers
+// during the scans. A 64-lane workgroup covers 8 x (4 / PW) blocks.
+template <int M, int PW>
+__global__ __launch_bounds__(64, 2) void k_match_temporal(MatchArgs a) {
+    extern __shared__ int2 kept[];              // [32 / PW blocks][2 directions][radius * ps_num] (pos, error bits)
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const int bl = lane & 31, dir = lane < 32 ? -1 : 1;
-    const int bx = tx * 8 + (bl & 7), by = ty * 4 + (bl >> 3);
+    const int win = lane % PW, dsel = (lane / PW) & 1, bl = lane / (2 * PW);
+    const int dir = dsel ? 1 : -1;
+    const int bx = tx * 8 + (bl & 7), by = ty * (4 / PW) + (bl >> 3);
     const bool valid = bx < a.nbx && by < a.nby;
     const int W = a.width, H = a.height, S = a.stride, Rt = a.radius;
     const int x = min(min(bx, a.nbx - 1) * a.block_step, W - 8);
@@ -917,23 +917,36 @@ __global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
     ref_terms<M>(R);
 
     int4 *out = a.matches + ((long long)min(by, a.nby - 1) * a.nbx + min(bx, a.nbx - 1)) * 8;
-    List8 L;
+    unsigned lpos[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int4 m = out[k];
-        L.er[k] = __int_as_float(m.w);
-        L.pos[k] = pack_xy(m.x, m.y);
-        L.zs[k] = Rt;
+        lpos[k] = pack_xy(m.x, m.y);
     }
-    unsigned lpos[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) lpos[k] = L.pos[k];
-    int4 *mine = fwd + bl * (Rt * a.ps_num);
+    const int per_dir = Rt * a.ps_num;
+    int2 *mine = kept + (bl * 2 + dsel) * per_dir;
     for (int t = 1; t <= Rt; ++t) {
         const int tz = Rt + dir * t;
         List8 F;
         list_init<false>(F, tz);
-        if (a.ps_range == 4) {
+        if constexpr (PW == 2) {  // ps_num == 2: this lane's window only
+            const unsigned c = win ? lpos[1] : lpos[0];
+            if (a.ps_range == 4) scan_band4<M>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(c), pos_y(c), tz, F);
+            else scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(c), pos_y(c), a.ps_range, tz, F);
+            float oe[8];
+            unsigned op[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                oe[k] = __shfl_xor(F.er[k], 1);
+                op[k] = (unsigned)__shfl_xor((int)F.pos[k], 1);
+            }
+            if (win == 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) insert8<false>(F, oe[k], op[k], tz);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) F.pos[k] = (unsigned)__shfl((int)F.pos[k], lane & ~1);
+        } else if (a.ps_range == 4) {
             for (int i = 0; i < a.ps_num; ++i)
                 scan_band4<M>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]), tz, F);
         } else {
@@ -943,20 +956,31 @@ __global__ __launch_bounds__(64) void k_match_temporal(MatchArgs a) {
                     scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]),
                                           a.ps_range, tz, F);
         }
+        if (win == 0) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if (i >= a.ps_num) continue;
-            if (dir < 0) insert8<true>(L, F.er[i], F.pos[i], tz);
-            else mine[(t - 1) * a.ps_num + i] = make_int4(pos_x(F.pos[i]), pos_y(F.pos[i]), tz, __float_as_int(F.er[i]));
+            for (int i = 0; i < 8; ++i)
+                if (i < a.ps_num) mine[(t - 1) * a.ps_num + i] = make_int2((int)F.pos[i], __float_as_int(F.er[i]));
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) lpos[k] = F.pos[k];
     }
     __syncthreads();
-    if (dir > 0 || !valid) return;
-    for (int e = 0; e < Rt * a.ps_num; ++e) {
-        const int4 m = mine[e];
-        insert8<true>(L, __int_as_float(m.w), pack_xy(m.x, m.y), m.z);
+    if (dsel != 0 || win != 0 || !valid) return;
+
+    List8 L;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int4 m = out[k];
+        L.er[k] = __int_as_float(m.w);
+        L.pos[k] = pack_xy(m.x, m.y);
+        L.zs[k] = Rt;
+    }
+    for (int d = 0; d < 2; ++d) {              // backward frames first, then forward
+        const int2 *e = kept + (bl * 2 + d) * per_dir;
+        for (int j = 0; j < per_dir; ++j) {
+            const int tz = Rt + (d == 0 ? -1 : 1) * (j / a.ps_num + 1);
+            insert8<true>(L, __int_as_float(e[j].y), (unsigned)e[j].x, tz);
+        }
     }
 
     const unsigned self = pack_xy(x, y);
@@ -1614,9 +1638,15 @@ template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
     launch_match_m<M>(c, false, s);
     MatchArgs t = a;
     t.tiles_x = (a.nbx + 7) / 8;
-    const dim3 grid((unsigned)(t.tiles_x * ((a.nby + 3) / 4)), 1, 1);
-    const size_t lds = (size_t)32 * a.radius * a.ps_num * sizeof(int4);
-    hipLaunchKernelGGL((k_match_temporal<M>), grid, dim3(64), lds, s, t);
+    if (a.ps_num == 2) {  // one lane per window
+        const dim3 grid((unsigned)(t.tiles_x * ((a.nby + 1) / 2)), 1, 1);
+        const size_t lds = (size_t)16 * 2 * a.radius * a.ps_num * sizeof(int2);
+        hipLaunchKernelGGL((k_match_temporal<M, 2>), grid, dim3(64), lds, s, t);
+    } else {
+        const dim3 grid((unsigned)(t.tiles_x * ((a.nby + 3) / 4)), 1, 1);
+        const size_t lds = (size_t)32 * 2 * a.radius * a.ps_num * sizeof(int2);
+        hipLaunchKernelGGL((k_match_temporal<M, 1>), grid, dim3(64), lds, s, t);
+    }
     return true;
 }
 
