@@ -329,7 +329,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the BM3D path shards by frame or plane, see DESIGN.md)", file=sys.stderr)
+              "(the BM3D path would shard by frame, plane or row band)", file=sys.stderr)
         sys.exit(2)
 
     import torch
