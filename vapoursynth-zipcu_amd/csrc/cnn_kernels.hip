@@ -24,6 +24,7 @@ namespace vszcnn {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CH = 32;
@@ -284,7 +285,8 @@ __global__ __launch_bounds__(256) void k_pack_weights(const float *__restrict__ 
 // 16 x 16 tile; the weights are wave-uniform, so they stay in scalar registers.
 
 // x1[n][y][x][o] = relu(b[o] + sum_t w[t][o][0] * bf16(noisy(p + t)) + w[t][o][1] * bf16(sigma) [p + t inside])
-// w0: fp32 [9][2][32] holding bf16-rounded values.
+// w0: fp32 [9][2][32] holding bf16-rounded values, then [32] sums over t of the sigma channel's
+// weights: a pixel whose 9 taps are all inside the image takes the sigma term as one product.
 __global__ __launch_bounds__(256) void k_conv_first(const float *__restrict__ noisy, int H, int W, int stride,
                                                     const float *__restrict__ sigma_map, const float *__restrict__ w0,
                                                     const float *__restrict__ b0, unsigned short *__restrict__ y) {
@@ -302,18 +304,30 @@ __global__ __launch_bounds__(256) void k_conv_first(const float *__restrict__ no
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx = x0 + tx, gy = y0 + ty;
     const float sg = (float)(__bf16)sigma_map[n];
+    const float *wsum = w0 + 9 * 2 * CH;
     float acc[CH];
+    if (gx >= 1 && gx + 1 < W && gy >= 1 && gy + 1 < H) {
 #pragma unroll
-    for (int o = 0; o < CH; ++o) acc[o] = b0[o];
+        for (int o = 0; o < CH; ++o) acc[o] = fmaf(wsum[o], sg, b0[o]);
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const float v = tile[ty + t / 3][tx + t % 3];
-        const bool in = v == v;
-        const float xv = in ? v : 0.0f, sv = in ? sg : 0.0f;
+        for (int t = 0; t < 9; ++t) {
+            const float v = tile[ty + t / 3][tx + t % 3];
 #pragma unroll
-        for (int o = 0; o < CH; ++o) {
-            acc[o] = fmaf(w0[(t * 2 + 0) * CH + o], xv, acc[o]);
-            acc[o] = fmaf(w0[(t * 2 + 1) * CH + o], sv, acc[o]);
+            for (int o = 0; o < CH; ++o) acc[o] = fmaf(w0[(t * 2 + 0) * CH + o], v, acc[o]);
+        }
+    } else {
+#pragma unroll
+        for (int o = 0; o < CH; ++o) acc[o] = b0[o];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const float v = tile[ty + t / 3][tx + t % 3];
+            const bool in = v == v;
+            const float xv = in ? v : 0.0f, sv = in ? sg : 0.0f;
+#pragma unroll
+            for (int o = 0; o < CH; ++o) {
+                acc[o] = fmaf(w0[(t * 2 + 0) * CH + o], xv, acc[o]);
+                acc[o] = fmaf(w0[(t * 2 + 1) * CH + o], sv, acc[o]);
+            }
         }
     }
     if (gx >= W || gy >= H) return;
@@ -326,9 +340,10 @@ __global__ __launch_bounds__(256) void k_conv_first(const float *__restrict__ no
 }
 
 // out[n][y][x] (row stride ostride) = noisy[n][y][x] + b5[0] + sum_t sum_i w5[t][i] * x[p + t][i]
-// w5: fp32 [9][32] holding bf16-rounded values; noisy: fp32 [n][H][stride].
+// w5: [9][16] bf16 pairs (channels 2k, 2k+1), the layout of two adjacent activation channels, so
+// each pair is one v_dot2_f32_bf16; noisy: fp32 [n][H][stride].
 __global__ __launch_bounds__(256) void k_conv_last(const unsigned short *__restrict__ x, int H, int W,
-                                                   const float *__restrict__ w5, const float *__restrict__ b5,
+                                                   const unsigned *__restrict__ w5, const float *__restrict__ b5,
                                                    const float *__restrict__ noisy, int stride,
                                                    float *__restrict__ out, int ostride) {
     __shared__ __attribute__((aligned(16))) char lds[HALO * HALO * PIX_BYTES];
@@ -346,29 +361,33 @@ __global__ __launch_bounds__(256) void k_conv_last(const unsigned short *__restr
             const uint4 v = *reinterpret_cast<const uint4 *>(px + part * 16);
             const unsigned u[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int i = part * 8 + 2 * k;
-                acc = fmaf(w5[t * CH + i], __uint_as_float(u[k] << 16), acc);
-                acc = fmaf(w5[t * CH + i + 1], __uint_as_float(u[k] & 0xffff0000u), acc);
-            }
+            for (int k = 0; k < 4; ++k)
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w5[t * (CH / 2) + part * 4 + k]),
+                                                      __builtin_bit_cast(bf16x2, u[k]), acc, false);
         }
     }
     if (gx >= W || gy >= H) return;
     out[((long long)n * H + gy) * ostride + gx] = noisy[((long long)n * H + gy) * stride + gx] + (acc + b5[0]);
 }
 
-// fp32 copies (bf16-rounded) of the end layers' weights from the packed forward layout.
+// The end layers' weights from the packed forward layout: fp32 copies (bf16-rounded) of the first
+// layer's, the sums over taps of its sigma channel, and the last layer's as bf16 pairs.
 __global__ __launch_bounds__(256) void k_end_weights(const unsigned short *__restrict__ wf0,
                                                      const unsigned short *__restrict__ wf5, float *__restrict__ w0,
-                                                     float *__restrict__ w5) {
+                                                     unsigned *__restrict__ w5) {
     const int i = threadIdx.x + blockIdx.x * 256;
     if (i < 9 * 2 * CH) {  // w0[t][i][o] = wf0[t][o][i], i in {0, 1}
         const int t = i / (2 * CH), ii = (i / CH) % 2, o = i % CH;
         w0[i] = bf2f(wf0[(t * CH + o) * CH + ii]);
     }
-    if (i < 9 * CH) {      // w5[t][i] = wf5[t][0][i]
-        const int t = i / CH, ii = i % CH;
-        w5[i] = bf2f(wf5[(t * CH + 0) * CH + ii]);
+    if (i < CH) {          // wsum[o] = sum over t of wf0[t][o][1]
+        float sum = 0.0f;
+        for (int t = 0; t < 9; ++t) sum += bf2f(wf0[(t * CH + i) * CH + 1]);
+        w0[9 * 2 * CH + i] = sum;
+    }
+    if (i < 9 * CH / 2) {  // w5[t][k] = (wf5[t][0][2k], wf5[t][0][2k+1])
+        const int t = i / (CH / 2), k = i % (CH / 2);
+        w5[i] = (unsigned)wf5[(t * CH + 0) * CH + 2 * k] | ((unsigned)wf5[(t * CH + 0) * CH + 2 * k + 1] << 16);
     }
 }
 
@@ -414,8 +433,8 @@ int vsz_cnn_run(const vsz_cnn_model *m, const float *noisy, int n, int h, int wd
     const size_t px = (size_t)n * h * wd;
     char *wk = static_cast<char *>(work);
     void *act[2] = {wk, wk + px * CH * 2};
-    float *w0 = reinterpret_cast<float *>(wk + 2 * px * CH * 2);   // [9][2][32]
-    float *w5 = w0 + 9 * 2 * CH;                                     // [9][32]
+    float *w0 = reinterpret_cast<float *>(wk + 2 * px * CH * 2);   // [9][2][32], then [32] sigma-tap sums
+    unsigned *w5 = reinterpret_cast<unsigned *>(w0 + 9 * 2 * CH + CH);   // [9][16] bf16 pairs
     hipLaunchKernelGGL(k_end_weights, dim3(3), dim3(256), 0, s, (const unsigned short *)m->w[0],
                        (const unsigned short *)m->w[5], w0, w5);
     const dim3 grid((wd + TILE - 1) / TILE, (h + TILE - 1) / TILE, n);
