@@ -1722,8 +1722,9 @@ int guard_bits_for(int tbx, int tby) {
 
 // Picks the block tile of the LDS-accumulating filter kernel: the region a tile's groups can
 // reach must fit the LDS budget; among those, the tiling with the least flush traffic (tiles x
-// region cells) that still yields enough workgroups to fill the chip.
-TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L, int min_wgs = 1024) {
+// region cells) that still yields enough workgroups to fill the chip once (two workgroups per CU
+// on 256 CUs; more, smaller tiles measured slower on 1080p planes at block_step 4 and 8).
+TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L, int min_wgs = 512) {
     TilePlan best;
     if (p->radius != 0) return best;
     const int opts[] = {32, 16, 8, 4, 2, 1};
