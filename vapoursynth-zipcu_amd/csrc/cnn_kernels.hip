@@ -38,16 +38,25 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 
 __device__ __forceinline__ float bf2f(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
 
-// Stages the (TILE+2)^2 input halo of image n (zeros outside the image) into LDS.
+// Stages the (TILE+2)^2 input halo of image n (zeros outside the image) into LDS. Every load of a
+// thread is issued before the first store, so the loads' latencies overlap.
 __device__ __forceinline__ void stage_halo(const unsigned short *x, int n, int H, int W, int x0, int y0, char *lds) {
-    for (int i = threadIdx.x; i < HALO * HALO * 4; i += 256) {
+    constexpr int kParts = HALO * HALO * 4, kPer = (kParts + 255) / 256;
+    uint4 v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
         const int pix = i >> 2, part = i & 3;
         const int hy = pix / HALO, hx = pix % HALO;
         const int gy = y0 + hy - 1, gx = x0 + hx - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-            v = *reinterpret_cast<const uint4 *>(x + (((long long)n * H + gy) * W + gx) * CH + part * 8);
-        *reinterpret_cast<uint4 *>(lds + pix * PIX_BYTES + part * 16) = v;
+        v[k] = make_uint4(0, 0, 0, 0);
+        if (i < kParts && gy >= 0 && gy < H && gx >= 0 && gx < W)
+            v[k] = *reinterpret_cast<const uint4 *>(x + (((long long)n * H + gy) * W + gx) * CH + part * 8);
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i < kParts) *reinterpret_cast<uint4 *>(lds + (i >> 2) * PIX_BYTES + (i & 3) * 16) = v[k];
     }
 }
 
