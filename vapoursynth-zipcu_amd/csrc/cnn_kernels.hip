@@ -301,13 +301,23 @@ __global__ __launch_bounds__(256) void k_conv_first(const float *__restrict__ no
                                                     const float *__restrict__ b0, unsigned short *__restrict__ y) {
     __shared__ float tile[HALO][HALO + 1];
     const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE, n = blockIdx.z;
-    for (int i = threadIdx.x; i < HALO * HALO; i += 256) {
+    // outside the image: a NaN marker, so the sigma term can be dropped there as well. Both loads
+    // of a thread are issued before its stores.
+    constexpr int kPer = (HALO * HALO + 255) / 256;
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
         const int hy = i / HALO, hx = i % HALO;
         const int gy = y0 + hy - 1, gx = x0 + hx - 1;
-        // outside the image: a NaN marker, so the sigma term can be dropped there as well
-        tile[hy][hx] = (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                           ? (float)(__bf16)noisy[((long long)n * H + gy) * stride + gx]
-                           : __builtin_nanf("");
+        v[k] = (i < HALO * HALO && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                   ? (float)(__bf16)noisy[((long long)n * H + gy) * stride + gx]
+                   : __builtin_nanf("");
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i < HALO * HALO) tile[i / HALO][i % HALO] = v[k];
     }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
