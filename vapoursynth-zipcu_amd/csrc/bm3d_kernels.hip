@@ -1334,6 +1334,8 @@ __device__ __forceinline__ void filter_tile_body(const FilterArgs &a, int tile_x
             const long long gid = (long long)(by0 + gs / nbx_t) * a.nbx + bx0 + gs % nbx_t;
             int mx[8], my[8], mz[8];
             load_matches(a.matches + gid * 8, mx, my, mz);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) mz[k] = 0;  // radius 0: every match is in slot 0
             float v[64];
             float r[FINAL ? 64 : 1];
             gather_group<FINAL>(a, p, mx, my, mz, c, v, r);
