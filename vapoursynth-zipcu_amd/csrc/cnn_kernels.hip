@@ -73,55 +73,52 @@ __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restric
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, g = lane >> 4;
 
-    bf16x8 wa[9][2];
+    // Wave w computes output channels hh*16 .. +15 (hh = w & 1) of rows (w >> 1)*8 .. +7: half the
+    // weights per lane, so more workgroups fit a CU and hide each other's staging latency.
+    const int hh = wave & 1, r0 = (wave >> 1) * 8;
+    bf16x8 wa[9];
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int hh = 0; hh < 2; ++hh)
-            wa[t][hh] = *reinterpret_cast<const bf16x8 *>(w + ((t * CH) + hh * 16 + j) * CH + g * 8);
+    for (int t = 0; t < 9; ++t) wa[t] = *reinterpret_cast<const bf16x8 *>(w + ((t * CH) + hh * 16 + j) * CH + g * 8);
 
     stage_halo(x, n, H, W, x0, y0, lds);
     __syncthreads();
+    if (mode == 1 && hh == 1) return;  // the residual output layer needs output channel 0 only
 
-#pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-        const int r = wave * 4 + rr;
-        f32x4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 2
+    for (int rr = 0; rr < 8; ++rr) {
+        const int r = r0 + rr;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int ky = t / 3, kx = t % 3;
             const bf16x8 b = *reinterpret_cast<const bf16x8 *>(lds + ((r + ky) * HALO + j + kx) * PIX_BYTES + g * 16);
-            acc[0] = mfma(wa[t][0], b, acc[0]);
-            acc[1] = mfma(wa[t][1], b, acc[1]);
+            acc = mfma(wa[t], b, acc);
         }
         const int gy = y0 + r, gx = x0 + j;
         if (gy >= H || gx >= W) continue;
         const long long pix = ((long long)n * H + gy) * W + gx;
         if (mode == 1) {
-            if (g == 0) static_cast<float *>(y)[((long long)n * H + gy) * ostride + gx] = base[pix] + (acc[0][0] + bias[0]);
+            if (g == 0) static_cast<float *>(y)[((long long)n * H + gy) * ostride + gx] = base[pix] + (acc[0] + bias[0]);
             continue;
         }
+        const int o = hh * 16 + g * 4;
+        float v[4];
 #pragma unroll
-        for (int hh = 0; hh < 2; ++hh) {
-            const int o = hh * 16 + g * 4;
-            float v[4];
+        for (int q = 0; q < 4; ++q) v[q] = acc[q];
+        if (mode == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = acc[hh][q];
-            if (mode == 0) {
+            for (int q = 0; q < 4; ++q) v[q] = fmaxf(v[q] + bias[o + q], 0.0f);
+        } else {
+            const uint2 m = *reinterpret_cast<const uint2 *>(aux + pix * CH + o);
+            const unsigned short mm[4] = {(unsigned short)(m.x & 0xffff), (unsigned short)(m.x >> 16),
+                                          (unsigned short)(m.y & 0xffff), (unsigned short)(m.y >> 16)};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = fmaxf(v[q] + bias[o + q], 0.0f);
-            } else {
-                const uint2 m = *reinterpret_cast<const uint2 *>(aux + pix * CH + o);
-                const unsigned short mm[4] = {(unsigned short)(m.x & 0xffff), (unsigned short)(m.x >> 16),
-                                              (unsigned short)(m.y & 0xffff), (unsigned short)(m.y >> 16)};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = bf2f(mm[q]) > 0.0f ? v[q] : 0.0f;
-            }
-            bf16x4 o4;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) o4[q] = (__bf16)v[q];
-            *reinterpret_cast<bf16x4 *>(static_cast<unsigned short *>(y) + pix * CH + o) = o4;
+            for (int q = 0; q < 4; ++q) v[q] = bf2f(mm[q]) > 0.0f ? v[q] : 0.0f;
         }
+        bf16x4 o4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o4[q] = (__bf16)v[q];
+        *reinterpret_cast<bf16x4 *>(static_cast<unsigned short *>(y) + pix * CH + o) = o4;
     }
 }
 
