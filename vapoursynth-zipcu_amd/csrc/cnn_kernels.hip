@@ -346,13 +346,22 @@ __global__ __launch_bounds__(256) void k_conv_first(const float *__restrict__ no
             }
         }
     }
-    if (gx >= W || gy >= H) return;
+    // The tile's 256 x 64 output bytes go through LDS so that consecutive lanes store consecutive
+    // 16-byte chunks of a tile row (one pixel per thread would store at a 64-byte stride).
+    __shared__ uint4 outs[TILE * TILE * 4];
     __bf16 h[CH];
 #pragma unroll
     for (int o = 0; o < CH; ++o) h[o] = (__bf16)fmaxf(acc[o], 0.0f);
-    uint4 *dst = reinterpret_cast<uint4 *>(y + (((long long)n * H + gy) * W + gx) * CH);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) dst[k] = reinterpret_cast<const uint4 *>(h)[k];
+    for (int k = 0; k < 4; ++k) outs[threadIdx.x * 4 + k] = reinterpret_cast<const uint4 *>(h)[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = (int)threadIdx.x + 256 * k, pix = idx >> 2;
+        const int oy = y0 + pix / TILE, ox = x0 + pix % TILE;
+        if (oy < H && ox < W)
+            reinterpret_cast<uint4 *>(y + (((long long)n * H + oy) * W + ox) * CH)[idx & 3] = outs[idx];
+    }
 }
 
 // out[n][y][x] (row stride ostride) = noisy[n][y][x] + b5[0] + sum_t sum_i w5[t][i] * x[p + t][i]
