@@ -293,15 +293,18 @@ def bench_learned4k(args):
     out = torch.empty_like(basic)
     stream = torch.cuda.current_stream()
     model = cnn.Model.load()
+    cm = model.cmodel()
     cwork = torch.empty(_lib_cnn_ws(1, H4, W4), dtype=torch.uint8, device="cuda")
+    sig_map = torch.full((1,), sig / 255.0, dtype=torch.float32, device="cuda")
 
     def plain():
         _fused_chain(kb, frames, None, work, acc, basic, r, stream)
         _fused_chain(kf, frames, basic, work, acc, out, r, stream)
 
     def learned():
-        for i in range(nf):
-            model.run(frames[i], sig, out=basic[i], work=cwork, stream=stream)
+        for i in range(nf):   # one CNN call per frame, noise map already on the device
+            cnn._check(cnn._lib.load().vsz_cnn_run(cnn.C.byref(cm), cnn._p(frames[i]), 1, H4, W4, W4, cnn._p(sig_map),
+                                                   cnn._p(basic[i]), cnn._p(cwork), cnn._s(stream)))
         _fused_chain(kf, frames, basic, work, acc, out, r, stream)
 
     res = {}
