@@ -444,7 +444,7 @@ int vsz_cnn_conv3x3(const void *x, const void *w, const float *bias, int n, int 
 size_t vsz_cnn_workspace_size(int n, int h, int wd) {
     if (n < 1 || h < 1 || wd < 1) return 0;
     const size_t px = (size_t)n * h * wd;
-    return 2 * px * CH * 2 + 4096;   // two activation buffers + the end layers' fp32 weights
+    return 2 * px * CH * 2 + 4096;   // two activation buffers + the end layers' weight tables (3 KB)
 }
 
 int vsz_cnn_run(const vsz_cnn_model *m, const float *noisy, int n, int h, int wd, int stride, const float *sigma_map,
