@@ -314,10 +314,12 @@ __device__ __forceinline__ void insert8(List8 &L, float e, unsigned pos, int z) 
             const bool fk = e < L.er[k];
             const bool fp = e < L.er[k - 1];
             if (fk) {
-                L.er[k] = fp ? L.er[k - 1] : e;
                 L.pos[k] = fp ? L.pos[k - 1] : pos;
                 if constexpr (TZ) L.zs[k] = fp ? L.zs[k - 1] : z;
             }
+            // er[k-1] <= er[k] and e is not NaN here, so the new entry is the median of the three
+            // (the same value as the selection above in every case, ties included)
+            L.er[k] = __builtin_amdgcn_fmed3f(L.er[k - 1], e, L.er[k]);
         }
         if (e < L.er[0]) {
             L.er[0] = e;
