@@ -321,11 +321,10 @@ __device__ __forceinline__ void insert8(List8 &L, float e, unsigned pos, int z) 
             // (the same value as the selection above in every case, ties included)
             L.er[k] = __builtin_amdgcn_fmed3f(L.er[k - 1], e, L.er[k]);
         }
-        if (e < L.er[0]) {
-            L.er[0] = e;
-            L.pos[0] = pos;
-            if constexpr (TZ) L.zs[0] = z;
-        }
+        const bool f0 = e < L.er[0];
+        L.pos[0] = f0 ? pos : L.pos[0];
+        if constexpr (TZ) L.zs[0] = f0 ? z : L.zs[0];
+        L.er[0] = fminf(L.er[0], e);  // neither is NaN; equal values on a tie
     }
 }
 
