@@ -324,7 +324,7 @@ __device__ __forceinline__ void insert8(List8 &L, float e, unsigned pos, int z) 
         const bool f0 = e < L.er[0];
         L.pos[0] = f0 ? pos : L.pos[0];
         if constexpr (TZ) L.zs[0] = f0 ? z : L.zs[0];
-        L.er[0] = fminf(L.er[0], e);  // neither is NaN; equal values on a tie
+        L.er[0] = f0 ? e : L.er[0];
     }
 }
 
