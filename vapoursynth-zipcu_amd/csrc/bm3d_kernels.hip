@@ -694,7 +694,7 @@ __device__ __forceinline__ void lds4_step(float (&w)[5][8], const lds_cptr (&pr)
     // left half ((c0 + c1) + (c2 + c3)) is the shifted right half, summed once
     const float half_sum = (own[0] + own[1]) + (own[2] + own[3]);
     const float e = dpp_from_left(half_sum) + half_sum;
-    if (ok) insert8<false>(L, e, pos, 0);
+    insert8<false>(L, ok ? e : FLT_MAX, pos, 0);  // FLT_MAX is never inserted: one branch per step
 }
 
 template <int M>
