@@ -305,7 +305,8 @@ template <bool TZ> __device__ __forceinline__ void list_init(List8 &L, int z) {
 }
 
 // Strict-< sorted insertion (src/bm3d.cu:641-653): a new entry goes before the first entry
-// with a strictly greater distance, so ties keep the earlier scan position in front.
+// with a strictly greater distance, so ties keep the earlier scan position in front. A distance
+// of FLT_MAX (how callers pass candidates outside the window) or NaN is never inserted.
 template <bool TZ>
 __device__ __forceinline__ void insert8(List8 &L, float e, unsigned pos, int z) {
     if (e < L.er[7]) {
@@ -362,7 +363,7 @@ __device__ __forceinline__ void scan_step(float (&w)[9][8], const PlaneBuf &b, c
                                           bool row_ok, unsigned ybits, int z, const RefPatch &R, List8 &L) {
     load_col<(U + 8) * 4>(w[(U + 8) % 9], b, vo);
     const float e = win_dist<M, U>(R, w);
-    if (row_ok && xt >= 0 && xt <= W - 8) insert8<TZ>(L, e, (unsigned)xt | ybits, z);
+    insert8<TZ>(L, row_ok && (unsigned)xt <= (unsigned)(W - 8) ? e : FLT_MAX, (unsigned)xt | ybits, z);
 }
 
 // Scans the window [cx-rng, cx+rng] x [cy-rng, cy+rng] of frame z, clipped to the plane, in the
@@ -482,7 +483,7 @@ __device__ __forceinline__ void scan_band4(const PlaneBuf &b, long long frame_of
                 for (int r = 0; r < 8; ++r) w9[c][r] = w[c][r];
             const float e = win_dist<M, 0>(R, w9);
             const int xt = x0 + t;
-            if (row_ok && xt >= 0 && xt <= W - 8) insert8<false>(F, e, (unsigned)xt | ybits, z);
+            insert8<false>(F, row_ok && (unsigned)xt <= (unsigned)(W - 8) ? e : FLT_MAX, (unsigned)xt | ybits, z);
         }
 #pragma unroll
         for (int r = 0; r < 7; ++r)
@@ -520,7 +521,7 @@ __device__ __forceinline__ void lds_step(float (&w)[9][8], const char *lds, cons
                                          bool row_ok, unsigned ybits, const RefPatch &R, List8 &L) {
     lds_col<(U + 8) * 4>(w[(U + 8) % 9], lds, vo);
     const float e = win_dist<M, U>(R, w);
-    if (row_ok && xt >= 0 && xt <= W - 8) insert8<false>(L, e, (unsigned)xt | ybits, 0);
+    insert8<false>(L, row_ok && (unsigned)xt <= (unsigned)(W - 8) ? e : FLT_MAX, (unsigned)xt | ybits, 0);
 }
 
 // WR wave rows: a workgroup of 128 * WR lanes. SPLIT lanes share a block: each scans a
