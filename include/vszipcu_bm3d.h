@@ -74,6 +74,13 @@ size_t vsz_bm3d_workspace_size(const vsz_bm3d_kparams *p);
  * block distance), ranks ascending. */
 int vsz_bm3d_match(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_matches, void *stream);
 
+/* Diagnostic, no reference counterpart: the matcher vsz_bm3d_match would launch for these
+ * parameters: "lds4" (shared-column, block_step 4), "lds_rows1/2/4" (LDS-staged, search rows split
+ * over 1, 2 or 4 lanes), "lds_short" (LDS-staged 16 x 8-block tile) or "global" (raw buffer loads,
+ * windows beyond the LDS budget); "+temporal" when radius > 0 (centre frame, then the predictive
+ * search). "" for invalid parameters. The returned string is static. */
+const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p);
+
 /* One full pass: matching, collaborative filtering and aggregation.
  *   d_clip0: [P][TW][H][stride]  basic estimate (final pass) or noisy clip (basic pass)
  *   d_clip1: [P][TW][H][stride]  noisy clip (final pass only, else ignored)

@@ -152,10 +152,10 @@ def test_shared_column_matcher_block_step4(orc, torch_gpu, W, H, rng, metric):
 
 
 @pytest.mark.parametrize("W,H,bs,rng,metric", [(200, 150, 8, 9, "ssd"), (333, 121, 8, 9, "zsad"), (140, 260, 7, 12, "ssd")])
-def test_lds_matcher_short_tiles(orc, torch_gpu, W, H, bs, rng, metric):
-    """Block steps whose 16 x 16-block search region exceeds the LDS budget (the plugin default
-    block_step 8 with bm_range 9) take the 16 x 8-block tile of the LDS-staged matcher."""
-    print(f"short-tile matcher: {W}x{H} bs={bs} R={rng} {metric}", flush=True)
+def test_lds_matcher_split_rows(orc, torch_gpu, W, H, bs, rng, metric):
+    """Small planes at other block steps (the plugin default block_step 8 with bm_range 9) take the
+    LDS-staged matcher with each block's search rows split over 4 lanes."""
+    print(f"split-row matcher: {W}x{H} bs={bs} R={rng} {metric}", flush=True)
     torch = torch_gpu
     img = _noisy(W, H, W + 5 * H)
     kp, op = kp_and_op(orc, W, H, bs=bs, rng=rng, metric=metric, sigma=device.scaled_sigma([10.0] * 3, False))

@@ -57,3 +57,67 @@ def test_split_bf16_against_fp64(orc, torch_gpu, W, H, final, seed):
 def test_split_bf16_needs_the_dct_pair():
     with pytest.raises(RuntimeError):
         device.num_blocks(device.kparams(64, 48, t2d="haar", precision="split-bf16"))
+
+
+def _stack(torch, kp, src):
+    """One pass at radius > 0 -> (the [1][TW][2][H][S] FP32 stack, match lists)."""
+    c0 = torch.from_numpy(np.ascontiguousarray(src[0])).cuda()
+    c1 = torch.from_numpy(np.ascontiguousarray(src[1])).cuda() if kp.final_pass else None
+    work = torch.zeros(device.workspace_size(kp), dtype=torch.uint8, device="cuda")
+    TW = 2 * kp.radius + 1
+    out = torch.zeros((1, TW, 2, kp.height, kp.stride), dtype=torch.float32, device="cuda")
+    device.run(kp, c0, c1, work, out, 1)
+    torch.cuda.synchronize()
+    nbx, nby = device.num_blocks(kp)
+    m = work[: nbx * nby * 128].view(torch.int32).reshape(nby, nbx, 8, 4).cpu().numpy()
+    return out, m
+
+
+@pytest.mark.gpu
+def test_split_bf16_temporal_radius2_against_fp64(orc, torch_gpu):
+    """The temporal workload's form at reduced size: 7 frames of 256x144 Gray with motion, sigma 8,
+    basic pass at radius 2 (block_step 8, bm_range 9, ps 2/4) into one accumulator stack per centre
+    frame, then VAggregate. The FP64 oracle runs on the same match lists (its own accumulation and
+    an FP64 VAggregate over the same aggZ slots)."""
+    torch = torch_gpu
+    W, H, F, r = 256, 144, 7, 2
+    TW = 2 * r + 1
+    print(f"split-bf16 temporal: {F} frames {W}x{H} r={r}", flush=True)
+    clean, nz = synth.moving_sequence(W, H, F, 8.0, seed=12)
+    sig = device.scaled_sigma([8.0] * 3, False)
+    kw = dict(block_step=8, bm_range=9, radius=r, ps_num=2, ps_range=4, sigma=tuple(sig), extractor_exp=3)
+    stacks = {"fp32": [], "split-bf16": [], "fp64": []}
+    for n in range(F):
+        win = np.stack([nz[min(max(n - r + t, 0), F - 1)] for t in range(TW)])[None, None]   # [1, 1, TW, H, W]
+        s32, m32 = _stack(torch, device.kparams(W, H, precision="fp32", **kw), win)
+        s16, m16 = _stack(torch, device.kparams(W, H, precision="split-bf16", **kw), win)
+        assert np.array_equal(m32, m16)
+        stacks["fp32"].append(s32[0])
+        stacks["split-bf16"].append(s16[0])
+        po = orc.make_params(W, H, threads=0, **kw)
+        r64, _, _ = orc.bm3d(po, win.astype(np.float64), dtype=np.float64,
+                             given_matches=np.ascontiguousarray(m32[..., :3]))
+        stacks["fp64"].append(r64[0])
+    errs = {}
+    for n in range(F):
+        idx = [min(max(n - r + i, 0), F - 1) for i in range(TW)]
+        zs = [orc.agg_z(i, n, F, r) for i in range(TW)]
+        num = sum(stacks["fp64"][c][z, 0] for c, z in zip(idx, zs))
+        den = sum(stacks["fp64"][c][z, 1] for c, z in zip(idx, zs))
+        want = num / den
+        for form in ("fp32", "split-bf16"):
+            dst = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+            device.vaggregate(W, r, n, F, [stacks[form][c] for c in idx], dst)
+            torch.cuda.synchronize()
+            got = dst.cpu().numpy().astype(np.float64)
+            e = np.abs(got - want)
+            mx, mn = errs.get(form, (0.0, 0.0))
+            errs[form] = (max(mx, float(e.max())), mn + float(e.mean()) / F)
+            if n == F // 2:
+                print(f"  frame {n} {form}: PSNR {synth.psnr(got, clean[n]):.4f} dB", flush=True)
+    for form, (mx, mn) in errs.items():
+        print(f"  {form} vs FP64: max-abs {mx:.3g} mean-abs {mn:.3g}", flush=True)
+    # FP32 rounding alone can flip the hard-threshold decision of a coefficient within an ulp of
+    # sigma, which moves a few pixels by up to ~1e-4; the split form flips more of them (2^-16)
+    assert errs["fp32"][0] <= 1e-3 and errs["fp32"][1] <= 1e-5
+    assert errs["split-bf16"][0] <= MAX_ABS and errs["split-bf16"][1] <= MEAN_ABS

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for h in ("vszipcu_bm3d.h", "vszipcu_cnn.h"):
         header = open(os.path.join(REPO, "include", h)).read()
         names |= set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(vsz_\w+)\s*\(", header, re.M))
-    assert len(names) == 29
+    assert len(names) == 30
     for n in sorted(names):
         assert hasattr(L, n), n
     assert set(_lib.PROTOTYPES) == names
@@ -149,3 +149,24 @@ def test_kernel_params_geometry_matches_oracle(orc):
     for (w, h, bs) in [(640, 360, 8), (1920, 1080, 4), (37, 29, 3), (8, 8, 1)]:
         kp = device.kparams(w, h, block_step=bs)
         assert device.num_blocks(kp) == orc.num_blocks(orc.make_params(w, h, block_step=bs))
+
+
+@pytest.mark.parametrize("kw,plan", [
+    (dict(width=1920, height=1080, block_step=4, bm_range=16), "lds4"),
+    (dict(width=1920, height=1080, block_step=4, bm_range=16, metric="zssd"), "lds_rows2"),
+    (dict(width=640, height=360, block_step=4, bm_range=9, metric="sad", radius=0), "lds4"),
+    (dict(width=640, height=360, block_step=5, bm_range=9), "lds_rows4"),
+    (dict(width=200, height=150, block_step=8, bm_range=9), "lds_rows4"),
+    (dict(width=3840, height=2160, block_step=7, bm_range=9), "lds_short"),
+    (dict(width=1920, height=1080, block_step=3, bm_range=9, metric="zsad"), "lds_rows1"),
+    (dict(width=200, height=150, block_step=8, bm_range=48), "global"),
+    (dict(width=120, height=96, block_step=8, bm_range=48, radius=1), "global+temporal"),
+    (dict(width=264, height=264, block_step=1, bm_range=57, radius=1), "global+temporal"),
+    (dict(width=3840, height=2160, block_step=8, bm_range=9, radius=1), "lds_rows2+temporal"),
+    (dict(width=72, height=64, block_step=4, bm_range=16, radius=2), "lds4+temporal"),
+])
+def test_match_plan_selects_the_parity_tested_kernels(kw, plan):
+    """The GPU parity cases of the global-memory and short-tile matchers rely on these choices."""
+    from vszipcu import device
+    w, h = kw.pop("width"), kw.pop("height")
+    assert device.match_plan(device.kparams(w, h, **kw)) == plan

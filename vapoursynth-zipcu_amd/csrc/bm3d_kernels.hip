@@ -357,13 +357,48 @@ template <int IMM> __device__ __forceinline__ void load_col(float (&col)[8], con
 }
 
 // One step of a search row: issue the load of the column the NEXT step needs (it has the whole
-// distance computation to arrive), then evaluate this step's candidate.
-template <int M, bool TZ, int U>
+// distance computation to arrive), then evaluate this step's candidate. vo[r] is the byte offset
+// of row r at the group's first column xg = xt - U. With CLAMP (xg < 0) the column is addressed
+// with its whole offset in the VGPR, clamped to the row start: in the first image row of the
+// stack vo[r] is negative there, and a buffer load does not bring a negative VGPR offset back into
+// range with its immediate offset (the load returns 0). Columns left of the row belong only to
+// candidates that are never inserted.
+template <int M, bool TZ, int U, bool CLAMP>
 __device__ __forceinline__ void scan_step(float (&w)[9][8], const PlaneBuf &b, const int (&vo)[8], int W, int xt,
                                           bool row_ok, unsigned ybits, int z, const RefPatch &R, List8 &L) {
-    load_col<(U + 8) * 4>(w[(U + 8) % 9], b, vo);
+    if constexpr (CLAMP) {
+        const int sh = 4 * max(U + 8, U - xt);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) w[(U + 8) % 9][r] = buf_load<0>(b, vo[r] + sh);
+    } else {
+        load_col<(U + 8) * 4>(w[(U + 8) % 9], b, vo);
+    }
     const float e = win_dist<M, U>(R, w);
     insert8<TZ>(L, row_ok && (unsigned)xt <= (unsigned)(W - 8) ? e : FLT_MAX, (unsigned)xt | ybits, z);
+}
+
+// Nine steps of a search row from column xg = x0 + tb (fewer at the row's end).
+template <int M, bool TZ, bool CLAMP>
+__device__ __forceinline__ void scan_group(float (&w)[9][8], const PlaneBuf &b, const int (&vo)[8], int W, int xg,
+                                           int left, bool row_ok, unsigned ybits, int z, const RefPatch &R,
+                                           List8 &L) {
+    scan_step<M, TZ, 0, CLAMP>(w, b, vo, W, xg + 0, row_ok, ybits, z, R, L);
+    if (left < 1) return;
+    scan_step<M, TZ, 1, CLAMP>(w, b, vo, W, xg + 1, row_ok, ybits, z, R, L);
+    if (left < 2) return;
+    scan_step<M, TZ, 2, CLAMP>(w, b, vo, W, xg + 2, row_ok, ybits, z, R, L);
+    if (left < 3) return;
+    scan_step<M, TZ, 3, CLAMP>(w, b, vo, W, xg + 3, row_ok, ybits, z, R, L);
+    if (left < 4) return;
+    scan_step<M, TZ, 4, CLAMP>(w, b, vo, W, xg + 4, row_ok, ybits, z, R, L);
+    if (left < 5) return;
+    scan_step<M, TZ, 5, CLAMP>(w, b, vo, W, xg + 5, row_ok, ybits, z, R, L);
+    if (left < 6) return;
+    scan_step<M, TZ, 6, CLAMP>(w, b, vo, W, xg + 6, row_ok, ybits, z, R, L);
+    if (left < 7) return;
+    scan_step<M, TZ, 7, CLAMP>(w, b, vo, W, xg + 7, row_ok, ybits, z, R, L);
+    if (left < 8) return;
+    scan_step<M, TZ, 8, CLAMP>(w, b, vo, W, xg + 8, row_ok, ybits, z, R, L);
 }
 
 // Scans the window [cx-rng, cx+rng] x [cy-rng, cy+rng] of frame z, clipped to the plane, in the
@@ -406,25 +441,15 @@ __device__ __forceinline__ void scan_window(const PlaneBuf &b, long long frame_o
                 for (int r = 0; r < 8; ++r) w[j][r] = buf_load<0>(b, vo[r] + dj);
             }
         }
-        for (int tb = 0; tb <= span; tb += 9) {
-            const int xt = x0 + tb;
-            scan_step<M, TZ, 0>(w, b, vo, W, xt + 0, row_ok, ybits, z, R, L);
-            if (tb + 1 > span) break;
-            scan_step<M, TZ, 1>(w, b, vo, W, xt + 1, row_ok, ybits, z, R, L);
-            if (tb + 2 > span) break;
-            scan_step<M, TZ, 2>(w, b, vo, W, xt + 2, row_ok, ybits, z, R, L);
-            if (tb + 3 > span) break;
-            scan_step<M, TZ, 3>(w, b, vo, W, xt + 3, row_ok, ybits, z, R, L);
-            if (tb + 4 > span) break;
-            scan_step<M, TZ, 4>(w, b, vo, W, xt + 4, row_ok, ybits, z, R, L);
-            if (tb + 5 > span) break;
-            scan_step<M, TZ, 5>(w, b, vo, W, xt + 5, row_ok, ybits, z, R, L);
-            if (tb + 6 > span) break;
-            scan_step<M, TZ, 6>(w, b, vo, W, xt + 6, row_ok, ybits, z, R, L);
-            if (tb + 7 > span) break;
-            scan_step<M, TZ, 7>(w, b, vo, W, xt + 7, row_ok, ybits, z, R, L);
-            if (tb + 8 > span) break;
-            scan_step<M, TZ, 8>(w, b, vo, W, xt + 8, row_ok, ybits, z, R, L);
+        // groups that start left of the plane first (lane-dependent count), then the rest
+        int tb = 0;
+        for (; tb <= span && x0 + tb < 0; tb += 9) {
+            scan_group<M, TZ, true>(w, b, vo, W, x0 + tb, span - tb, row_ok, ybits, z, R, L);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) vo[r] += 36;
+        }
+        for (; tb <= span; tb += 9) {
+            scan_group<M, TZ, false>(w, b, vo, W, x0 + tb, span - tb, row_ok, ybits, z, R, L);
 #pragma unroll
             for (int r = 0; r < 8; ++r) vo[r] += 36;
         }
@@ -805,84 +830,50 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
     match_lds4_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), region);
 }
 
-template <int M, bool TEMPORAL>
+// Spatial search on raw buffer loads, for windows whose staged region exceeds the LDS budget: one
+// lane per reference block, 8 x 8 blocks per wave.
+template <int M>
 __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
     const bool valid = bx < a.nbx && by < a.nby;
-    const int W = a.width, H = a.height, S = a.stride, Rt = TEMPORAL ? a.radius : 0;
+    const int W = a.width, H = a.height, S = a.stride;
     const int x = min(min(bx, a.nbx - 1) * a.block_step, W - 8);
     const int y = min(min(by, a.nby - 1) * a.block_step, H - 8);
-    const float *centre = a.img + (long long)Rt * a.ts;
-    const PlaneBuf pb = make_plane_buf(a.img, (unsigned)(a.ts * (2 * Rt + 1) * 4));
+    const PlaneBuf pb = make_plane_buf(a.img, (unsigned)(a.ts * 4));
 
     RefPatch R;
 #pragma unroll
     for (int r = 0; r < 8; ++r)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) R.a[r * 8 + c] = centre[(long long)(y + r) * S + x + c];
+        for (int c = 0; c < 8; ++c) R.a[r * 8 + c] = a.img[(long long)(y + r) * S + x + c];
     ref_terms<M>(R);
 
     List8 L;
-    list_init<TEMPORAL>(L, Rt);
-    scan_window<M, TEMPORAL>(pb, (long long)Rt * a.ts, S, W, H, R, x, y, a.bm_range, Rt, L);
+    list_init<false>(L, 0);
+    scan_window<M, false>(pb, 0, S, W, H, R, x, y, a.bm_range, 0, L);
 
-    if constexpr (TEMPORAL) {
-        // predictive search (src/bm3d.cu:663-756): around the first ps_num entries of the
-        // previous frame's list, starting from the centre frame's list in each direction
-        unsigned cpos[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) cpos[k] = L.pos[k];
-        for (int dir = -1; dir <= 1; dir += 2) {
-            unsigned lpos[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) lpos[k] = cpos[k];
-            for (int t = 1; t <= Rt; ++t) {
-                const int tz = Rt + dir * t;
-                List8 F;
-                list_init<false>(F, tz);
-                if (a.ps_range == 4) {
-                    for (int i = 0; i < a.ps_num; ++i)
-                        scan_band4<M>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]), pos_y(lpos[i]), tz, F);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 8; ++i)
-                        if (i < a.ps_num)
-                            scan_window<M, false>(pb, (long long)tz * a.ts, S, W, H, R, pos_x(lpos[i]),
-                                                  pos_y(lpos[i]), a.ps_range, tz, F);
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    if (i < a.ps_num) insert8<true>(L, F.er[i], F.pos[i], tz);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) lpos[k] = F.pos[k];
-            }
-        }
-    }
-
-    // self-inclusion (src/bm3d.cu:758-788)
+    // self-inclusion (src/bm3d.cu:758-788); a centre-frame search that a temporal stage follows
+    // (no_self) leaves it to that stage, which applies it after its merges
     const unsigned self = pack_xy(x, y);
-    bool found = false;
+    bool found = a.no_self != 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) found |= (L.pos[k] == self && (!TEMPORAL || L.zs[k] == Rt));
+    for (int k = 0; k < 8; ++k) found |= L.pos[k] == self;
     if (!found) {
 #pragma unroll
         for (int k = 7; k > 0; --k) {
             L.er[k] = L.er[k - 1];
             L.pos[k] = L.pos[k - 1];
-            if constexpr (TEMPORAL) L.zs[k] = L.zs[k - 1];
         }
         L.er[0] = 0.0f;
         L.pos[0] = self;
-        if constexpr (TEMPORAL) L.zs[0] = Rt;
     }
     if (!valid) return;
     int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), TEMPORAL ? L.zs[k] : 0, __float_as_int(L.er[k]));
+    for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
 }
 
 // Temporal matching after the centre frame's spatial search: reads the centre lists, runs the
@@ -1627,19 +1618,90 @@ size_t match_lds4_bytes(const MatchArgs &a, int *lds_stride) {
     return bytes <= kMatchLdsBudget ? bytes : 0;
 }
 
-template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s);
+// Which kernel runs a spatial (or centre-frame) search.
+enum MatchKernel { kMkLds4 = 0, kMkLds1, kMkLds2, kMkLds4Bands, kMkLdsShort, kMkGlobal };
 
-// Radius > 0: the centre frame's spatial search on an LDS-staged matcher, then the temporal
-// stage; false when neither LDS matcher fits (the single-kernel path then does everything).
-template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
+struct MatchPlan {
+    MatchKernel kernel;
+    size_t lds;
+    int lds_stride;
+};
+
+// The shared-column matcher for block_step 4 with ssd/sad; otherwise the LDS-staged matcher with
+// each block's search rows split over 2 or 4 lanes when the plane has too few blocks to fill the
+// chip (about 2048 waves), then its 16 x 8-block tile; the global-memory matcher when no staged
+// region fits the LDS budget.
+MatchPlan plan_match(const MatchArgs &a, int metric) {
+    MatchPlan m{kMkGlobal, 0, 0};
+    if (metric == 0 || metric == 1) {
+        m.lds = match_lds4_bytes(a, &m.lds_stride);
+        if (m.lds) {
+            m.kernel = kMkLds4;
+            return m;
+        }
+    }
+    const long long nb = (long long)a.nbx * a.nby;
+    const int split = nb >= 2048LL * 64 ? 1 : nb >= 1024LL * 64 ? 2 : 4;
+    m.lds = match_lds_bytes(a.block_step, a.bm_range, &m.lds_stride, 2, split);
+    if (m.lds) {
+        m.kernel = split == 1 ? kMkLds1 : split == 2 ? kMkLds2 : kMkLds4Bands;
+        return m;
+    }
+    m.lds = match_lds_bytes(a.block_step, a.bm_range, &m.lds_stride, 1, 1);
+    if (m.lds) {
+        m.kernel = kMkLdsShort;
+        return m;
+    }
+    m.lds = 0;
+    m.lds_stride = 0;
+    return m;
+}
+
+template <int M> void launch_spatial(const MatchArgs &a, const MatchPlan &pl, hipStream_t s) {
+    MatchArgs b = a;
+    b.lds_stride = pl.lds_stride;
+    switch (pl.kernel) {
+    case kMkLds4:
+        if constexpr (M == 0 || M == 1) {
+            b.tiles_x = (a.nbx + 14) / 15;
+            hipLaunchKernelGGL((k_match_lds4<M>), dim3((unsigned)(b.tiles_x * ((a.nby + 7) / 8))), dim3(256), pl.lds, s, b);
+        }
+        break;
+    case kMkLds1:
+        b.tiles_x = (a.nbx + 15) / 16;
+        hipLaunchKernelGGL((k_match_lds<M, 2, 1>), dim3((unsigned)(b.tiles_x * ((a.nby + 15) / 16))), dim3(256), pl.lds, s, b);
+        break;
+    case kMkLds2:
+        b.tiles_x = (a.nbx + 15) / 16;
+        hipLaunchKernelGGL((k_match_lds<M, 2, 2>), dim3((unsigned)(b.tiles_x * ((a.nby + 7) / 8))), dim3(256), pl.lds, s, b);
+        break;
+    case kMkLds4Bands:
+        b.tiles_x = (a.nbx + 15) / 16;
+        hipLaunchKernelGGL((k_match_lds<M, 2, 4>), dim3((unsigned)(b.tiles_x * ((a.nby + 3) / 4))), dim3(256), pl.lds, s, b);
+        break;
+    case kMkLdsShort:
+        b.tiles_x = (a.nbx + 15) / 16;
+        hipLaunchKernelGGL((k_match_lds<M, 1, 1>), dim3((unsigned)(b.tiles_x * ((a.nby + 7) / 8))), dim3(128), pl.lds, s, b);
+        break;
+    default:
+        b.tiles_x = (a.nbx + 7) / 8;
+        hipLaunchKernelGGL((k_match<M>), dim3((unsigned)(b.tiles_x * ((a.nby + 7) / 8))), dim3(64), 0, s, b);
+        break;
+    }
+}
+
+// radius 0: the spatial search with its self-inclusion. radius > 0: the centre frame's spatial
+// search with the self-inclusion held back, then the temporal stage (predictive search, merges,
+// self-inclusion).
+template <int M> void launch_match_m(const MatchArgs &a, hipStream_t s) {
+    if (a.radius == 0) {
+        launch_spatial<M>(a, plan_match(a, M), s);
+        return;
+    }
     MatchArgs c = a;
     c.img = a.img + (long long)a.radius * a.ts;
     c.no_self = 1;
-    int ls = 0;
-    const bool lds4 = (M == 0 || M == 1) && match_lds4_bytes(a, &ls) != 0;
-    if (!lds4 && !match_lds_bytes(a.block_step, a.bm_range, &ls, 2, 4) && !match_lds_bytes(a.block_step, a.bm_range, &ls, 1))
-        return false;
-    launch_match_m<M>(c, false, s);
+    launch_spatial<M>(c, plan_match(c, M), s);
     MatchArgs t = a;
     t.tiles_x = (a.nbx + 7) / 8;
     if (a.ps_num == 2) {  // one lane per window
@@ -1651,57 +1713,15 @@ template <int M> bool launch_match_split(const MatchArgs &a, hipStream_t s) {
         const size_t lds = (size_t)32 * 2 * a.radius * a.ps_num * sizeof(int2);
         hipLaunchKernelGGL((k_match_temporal<M, 1>), grid, dim3(64), lds, s, t);
     }
-    return true;
 }
 
-template <int M> void launch_match_m(const MatchArgs &a, bool temporal, hipStream_t s) {
-    if (temporal && launch_match_split<M>(a, s)) return;
-    if constexpr (M == 0 || M == 1) {
-        if (!temporal) {
-            MatchArgs b = a;
-            const size_t lds = match_lds4_bytes(a, &b.lds_stride);
-            if (lds) {
-                b.tiles_x = (a.nbx + 14) / 15;
-                const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 7) / 8)), 1, 1);
-                hipLaunchKernelGGL((k_match_lds4<M>), grid, dim3(256), lds, s, b);
-                return;
-            }
-        }
-    }
-    if (!temporal) {
-        // split each block's rows over 2 or 4 lanes when the plane has too few blocks to fill
-        // the chip (about 2048 waves)
-        const long long nb = (long long)a.nbx * a.nby;
-        const int split = nb >= 2048LL * 64 ? 1 : nb >= 1024LL * 64 ? 2 : 4;
-        MatchArgs b = a;
-        b.tiles_x = (a.nbx + 15) / 16;
-        size_t lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 2, split);
-        if (lds) {
-            const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 16 / split - 1) / (16 / split))), 1, 1);
-            if (split == 1) hipLaunchKernelGGL((k_match_lds<M, 2, 1>), grid, dim3(256), lds, s, b);
-            else if (split == 2) hipLaunchKernelGGL((k_match_lds<M, 2, 2>), grid, dim3(256), lds, s, b);
-            else hipLaunchKernelGGL((k_match_lds<M, 2, 4>), grid, dim3(256), lds, s, b);
-            return;
-        }
-        lds = match_lds_bytes(a.block_step, a.bm_range, &b.lds_stride, 1, 1);   // e.g. block_step 8
-        if (lds) {
-            const dim3 grid((unsigned)(b.tiles_x * ((a.nby + 7) / 8)), 1, 1);
-            hipLaunchKernelGGL((k_match_lds<M, 1, 1>), grid, dim3(128), lds, s, b);
-            return;
-        }
-    }
-    const dim3 grid((unsigned)(a.tiles_x * ((a.nby + 7) / 8)), 1, 1);
-    if (temporal) hipLaunchKernelGGL((k_match<M, true>), grid, dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_match<M, false>), grid, dim3(64), 0, s, a);
-}
-
-void launch_match(const MatchArgs &a, int metric, bool temporal, hipStream_t s) {
+void launch_match(const MatchArgs &a, int metric, hipStream_t s) {
     switch (metric) {
-    case 0: launch_match_m<0>(a, temporal, s); break;
-    case 1: launch_match_m<1>(a, temporal, s); break;
-    case 2: launch_match_m<2>(a, temporal, s); break;
-    case 3: launch_match_m<3>(a, temporal, s); break;
-    default: launch_match_m<4>(a, temporal, s); break;
+    case 0: launch_match_m<0>(a, s); break;
+    case 1: launch_match_m<1>(a, s); break;
+    case 2: launch_match_m<2>(a, s); break;
+    case 3: launch_match_m<3>(a, s); break;
+    default: launch_match_m<4>(a, s); break;
     }
 }
 
@@ -1857,8 +1877,29 @@ int vsz_bm3d_match(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_mat
     m.ps_range = p->ps_range;
     m.ts = L.ts;
     m.tiles_x = (L.nbx + 7) / 8;
-    launch_match(m, p->metric, p->radius > 0, (hipStream_t)stream);
+    launch_match(m, p->metric, (hipStream_t)stream);
     return check_launch("vsz_bm3d_match");
+}
+
+const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p) {
+    static const char *const names[] = {"lds4", "lds_rows1", "lds_rows2", "lds_rows4", "lds_short", "global"};
+    static const char *const temporal_names[] = {"lds4+temporal", "lds_rows1+temporal", "lds_rows2+temporal",
+                                                 "lds_rows4+temporal", "lds_short+temporal", "global+temporal"};
+    if (!kparams_ok(p)) {
+        vsz_fail(VSZ_EINVAL, "invalid kernel parameters");
+        return "";
+    }
+    const Layout L = layout_of(p);
+    MatchArgs m{};
+    m.width = p->width;
+    m.height = p->height;
+    m.stride = p->stride;
+    m.nbx = L.nbx;
+    m.nby = L.nby;
+    m.block_step = p->block_step;
+    m.bm_range = p->bm_range;
+    const MatchPlan pl = plan_match(m, p->metric);
+    return p->radius > 0 ? temporal_names[pl.kernel] : names[pl.kernel];
 }
 
 int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, void *d_work, float *d_out,

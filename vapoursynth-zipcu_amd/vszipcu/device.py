@@ -65,6 +65,14 @@ def num_blocks(p: _lib.KParams):
     return nbx.value, nby.value
 
 
+def match_plan(p: _lib.KParams) -> str:
+    """Name of the matcher vsz_bm3d_match launches for these parameters (no GPU needed)."""
+    name = _lib.load().vsz_bm3d_match_plan(C.byref(p)).decode()
+    if not name:
+        raise RuntimeError(f"vszipcu: {_lib.last_error()}")
+    return name
+
+
 def workspace_size(p: _lib.KParams) -> int:
     n = _lib.load().vsz_bm3d_workspace_size(C.byref(p))
     if n == 0:
