@@ -93,6 +93,17 @@ const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p);
 int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, void *d_work,
                  float *d_out, int out_kind, void *stream);
 
+/* vsz_bm3d_run without a workspace argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. The match lists and accumulators live in a static device array
+ * of 128 MiB (134217728 bytes) that each GPU holds from the library's first use until the process
+ * ends; a pass whose vsz_bm3d_workspace_size() exceeds it is rejected with VSZ_EINVAL. The call
+ * allocates nothing. Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race, so serialise them. Inputs, outputs and results are those of
+ * vsz_bm3d_run. */
+int vsz_bm3d_run_noalloc(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, float *d_out,
+                         int out_kind, void *stream);
+
 /* Temporal aggregation of one plane (src/bm3d_vagg.cu:18-48): d_dst[y][x] =
  * sum_i slot_i[z_i][0] / sum_i slot_i[z_i][1], i ascending, z_i = min(max(2r-i, n-nf+1+r), n+r).
  * d_slots[i]: [TW][2][H][stride] float32 stack of centre frame clamp(n-r+i, 0, nf-1). */

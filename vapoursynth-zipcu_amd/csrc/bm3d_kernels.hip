@@ -1902,11 +1902,76 @@ const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p) {
     return p->radius > 0 ? temporal_names[pl.kernel] : names[pl.kernel];
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of vsz_bm3d_run_noalloc: the match lists and the accumulator stack of one pass. Each GPU
+// holds its own copy from the library's first use on it until the process ends. Host code must
+// not pass `scratch` itself to a kernel (on the host it names a stand-in, not device memory); it
+// uses the device address scratch_ptr() looked up once per device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+
+// Device address of the current device's scratch, or nullptr (error set) on failure.
+void *scratch_ptr() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        vsz_fail(VSZ_EDEVICE, "vsz_bm3d_run_noalloc: no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            vsz_fail(VSZ_EDEVICE, "vsz_bm3d_run_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+int run_pass(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, void *d_work, float *d_out,
+             int out_kind, void *stream);
+
+}  // namespace
+
+extern "C" {
+
 int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, void *d_work, float *d_out,
                  int out_kind, void *stream) {
     if (!kparams_ok(p) || !d_clip0 || !d_work || !d_out || (p->final_pass && !d_clip1) || out_kind < 0 ||
         out_kind > 1 || (out_kind == 0 && p->radius != 0))
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_run: invalid arguments");
+    return run_pass(p, d_clip0, d_clip1, d_work, d_out, out_kind, stream);
+}
+
+int vsz_bm3d_run_noalloc(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, float *d_out,
+                         int out_kind, void *stream) {
+    if (!kparams_ok(p) || !d_clip0 || !d_out || (p->final_pass && !d_clip1) || out_kind < 0 || out_kind > 1 ||
+        (out_kind == 0 && p->radius != 0))
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_run_noalloc: invalid arguments");
+    const size_t need = vsz_bm3d_workspace_size(p);
+    if (need > kScratchBytes)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_run_noalloc: the pass needs %zu bytes of workspace, above the %zu-byte "
+                        "static scratch; use vsz_bm3d_run with a workspace", need, kScratchBytes);
+    void *work = scratch_ptr();
+    if (!work) return VSZ_EDEVICE;
+    return run_pass(p, d_clip0, d_clip1, work, d_out, out_kind, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+int run_pass(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_clip1, void *d_work, float *d_out,
+             int out_kind, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const Layout L = layout_of(p);
     int4 *matches = reinterpret_cast<int4 *>(d_work);
@@ -1979,6 +2044,10 @@ int vsz_bm3d_run(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d
     else hipLaunchKernelGGL(k_finish<false>, grid, dim3(256), 0, s, fa);
     return check_launch("vsz_bm3d_run: finish");
 }
+
+}  // namespace
+
+extern "C" {
 
 int vsz_vaggregate_run(int width, int height, int stride, int radius, int n, int num_frames,
                        const float *const *d_slots, float *d_dst, void *stream) {

@@ -101,6 +101,8 @@ PROTOTYPES = {
     "vsz_bm3d_match_plan": (C.c_char_p, [C.POINTER(KParams)]),
     "vsz_bm3d_run": (C.c_int, [C.POINTER(KParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                C.c_void_p]),
+    "vsz_bm3d_run_noalloc": (C.c_int, [C.POINTER(KParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_void_p]),
     "vsz_vaggregate_run": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]),
     "vsz_bm3d_create": (C.c_int, [C.POINTER(Bm3dArgs), C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
