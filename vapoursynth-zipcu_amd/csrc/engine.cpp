@@ -7,7 +7,7 @@
 //   upload / launch / download ...................... src/bm3d.zig:274-478
 //   fused accumulator chain ......................... src/bm3d.zig:481-633, 898-949
 //   stream-slot pool ................................ src/pool.zig
-//   device frame caches ............................. src/framecache.zig
+//   device slot caches (lease table) ................ src/framecache.zig
 //   CPU VAggregate .................................. src/bm3d.zig:1284-1363
 
 #include <hip/hip_runtime.h>
@@ -21,12 +21,15 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
+#include <new>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "internal.h"
+#include "slot_cache.h"
 #include "vszipcu_bm3d.h"
 
 // ---------------------------------------------------------------------------------------------
@@ -116,136 +119,55 @@ int device_stride(int w) { return (w + 63) & ~63; }
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // ---------------------------------------------------------------------------------------------
-// device frame cache (src/framecache.zig): refcounted keyed slots, all-or-nothing acquire.
+// Device slot cache: device buffers (and their fill events) managed by a LeaseTable (slot_cache.h).
 // ---------------------------------------------------------------------------------------------
-class FrameCache {
+class SlotCache {
   public:
-    struct Slot {
-        int64_t key = -1;
+    struct Buffer {
         void *buf = nullptr;
-        hipEvent_t ev = nullptr;
-        unsigned refs = 0;
-        uint64_t stamp = 0;
-        bool ready = false;
+        hipEvent_t ev = nullptr;  // recorded after the buffer was filled
     };
-    std::vector<Slot> slots;
-    size_t slot_bytes = 0;
+    using Lease = LeaseTable::Lease;
 
-    ~FrameCache() { clear(); }
+    ~SlotCache() { clear(); }
 
-    void clear() {
-        for (auto &s : slots) {
-            if (s.buf) (void)hipFree(s.buf);
-            if (s.ev) (void)hipEventDestroy(s.ev);
-        }
-        slots.clear();
-    }
-
-    // Allocates up to `want` slots; stops early (without error) on device OOM once `min_slots`
-    // exist (src/bm3d.zig:865-885).
-    int init(size_t bytes, size_t want, size_t min_slots) {
-        slot_bytes = bytes;
-        slots.reserve(want);
+    // Allocates up to `want` buffers of `bytes`; on device OOM keeps what it has if that is at
+    // least `min_bufs` (src/bm3d.zig:865-885, 923-935).
+    int init(size_t bytes, size_t want, size_t min_bufs) {
+        clear();
         for (size_t i = 0; i < want; ++i) {
-            Slot s;
-            hipError_t e = hipMalloc(&s.buf, std::max<size_t>(bytes, 256));
+            Buffer b;
+            const hipError_t e = hipMalloc(&b.buf, std::max<size_t>(bytes, 256));
             if (e != hipSuccess) {
                 (void)hipGetLastError();
-                if (e == hipErrorOutOfMemory && slots.size() >= min_slots) break;
+                if (e == hipErrorOutOfMemory && bufs_.size() >= min_bufs) break;
                 return e == hipErrorOutOfMemory ? VSZ_ENOMEM : VSZ_EDEVICE;
             }
-            if (hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) {
-                (void)hipFree(s.buf);
+            if (hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+                (void)hipFree(b.buf);
                 return VSZ_EDEVICE;
             }
-            slots.push_back(s);
+            bufs_.push_back(b);
         }
+        table_.reset(bufs_.size());
         return VSZ_OK;
     }
 
-    // Claims every key; load[i] = the caller must fill slot idx[i] and publish it.
-    void acquire(const int64_t *keys, int n, int *idx, bool *load) {
-        std::unique_lock<std::mutex> lk(mu_);
-        for (;;) {
-            bool pending = false;
-            for (int i = 0; i < n && !pending; ++i)
-                for (auto &s : slots)
-                    if (s.key == keys[i] && !s.ready) {
-                        pending = true;
-                        break;
-                    }
-            if (pending) {
-                cv_.wait(lk);
-                continue;
-            }
-            int claimed = 0;
-            bool ok = true;
-            for (; claimed < n; ++claimed) {
-                load[claimed] = false;
-                int found = -1;
-                for (size_t si = 0; si < slots.size(); ++si)
-                    if (slots[si].key == keys[claimed]) {
-                        found = int(si);
-                        break;
-                    }
-                if (found < 0) {
-                    int victim = -1;
-                    for (size_t si = 0; si < slots.size(); ++si) {
-                        if (slots[si].refs) continue;
-                        if (victim < 0 || slots[si].stamp < slots[victim].stamp) victim = int(si);
-                    }
-                    if (victim < 0) {
-                        for (int j = 0; j < claimed; ++j) {
-                            slots[idx[j]].refs -= 1;
-                            if (load[j]) slots[idx[j]].key = -1;
-                        }
-                        ok = false;
-                        break;
-                    }
-                    slots[victim].key = keys[claimed];
-                    slots[victim].ready = false;
-                    found = victim;
-                    load[claimed] = true;
-                }
-                idx[claimed] = found;
-                slots[found].refs += 1;
-                slots[found].stamp = ++clock_;
-            }
-            if (ok) return;
-            cv_.notify_all();
-            cv_.wait(lk);
+    void clear() {
+        for (auto &b : bufs_) {
+            if (b.buf) (void)hipFree(b.buf);
+            if (b.ev) (void)hipEventDestroy(b.ev);
         }
+        bufs_.clear();
+        table_.reset(0);
     }
 
-    void publish(int si) {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            slots[si].ready = true;
-        }
-        cv_.notify_all();
-    }
-
-    // Undo a claim that never published (a no-op on published slots).
-    void abandon(int si) {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            if (!slots[si].ready) slots[si].key = -1;
-        }
-        cv_.notify_all();
-    }
-
-    void release(const int *idx, int n) {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            for (int i = 0; i < n; ++i) slots[idx[i]].refs -= 1;
-        }
-        cv_.notify_all();
-    }
+    Lease lease(const int64_t *keys, int n) { return table_.lease(keys, n); }
+    Buffer &buffer(const Lease &l, int i) { return bufs_[l.slot(i)]; }
 
   private:
-    std::mutex mu_;
-    std::condition_variable cv_;
-    uint64_t clock_ = 0;
+    std::vector<Buffer> bufs_;
+    LeaseTable table_;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -379,8 +301,8 @@ class Bm3d final : public vsz_filter {
     size_t sum_src = 0, sum_res = 0, sum_dst = 0, sum_work = 0, cache_elems = 0;
     bool scache = false;
     StreamPool pool;
-    FrameCache cache;  // uploaded source frames, key = (clip << 40) | frame
-    FrameCache acc;    // fused: FP32 accumulator stacks per centre frame
+    SlotCache cache;  // uploaded source frames, key = (clip << 40) | frame
+    SlotCache acc;    // fused: FP32 accumulator stacks per centre frame
     std::string v2_error = "BM3D: process frame failed.";
 
     ~Bm3d() override {
@@ -415,7 +337,7 @@ class Bm3d final : public vsz_filter {
 
   private:
     int upload_window(StreamSlot &s, const int *idx, int count, vsz_get_plane_fn get, void *user,
-                      std::vector<int> &held, std::vector<bool> &loaded);
+                      SlotCache::Lease &lease);
     int launch_entries(StreamSlot &s, float *out_base, int out_kind);
     int download(StreamSlot &s, const float *d_base, bool stacked, void *const dst[3], const ptrdiff_t dst_stride[3]);
     int process_normal(StreamSlot &s, int n, vsz_get_plane_fn get, void *user, void *const dst[3],
@@ -664,16 +586,28 @@ int Bm3d::init_device(char *err, size_t errlen) {
         const size_t slot_bytes = sum_res * 4;
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return oom();
+        // VSZIPCU_FREE_MEM_CAP_MIB caps the free memory this sizing sees (a testing aid that makes
+        // the out-of-memory path below reachable on a large GPU)
+        if (const char *cap_env = std::getenv("VSZIPCU_FREE_MEM_CAP_MIB")) {
+            const long long cap_mib = std::atoll(cap_env);
+            if (cap_mib >= 0) free_b = std::min(free_b, size_t(cap_mib) << 20);
+        }
         const size_t margin = size_t(256) << 20;
         const size_t budget = free_b > margin ? free_b - margin : 0;
         const size_t cap = std::min(budget / std::max<size_t>(slot_bytes, 1), want);
         if (cap < min_slots || acc.init(slot_bytes, cap, min_slots)) {
+            // the reference formats into a 320-byte buffer and falls back to a fixed text when the
+            // message does not fit (src/bm3d.zig:1236-1246)
             char buf[320];
-            std::snprintf(buf, sizeof buf,
+            const int len = std::snprintf(buf, sizeof buf,
                           "BM3Dv2: fast_fused needs at least %zu MiB of accumulator cache (%zu slots x %zu MiB) "
                           "but only %zu MiB of device memory is free. Use fast_fused=False (composed BM3D + "
                           "VAggregate, much less VRAM), or lower num_streams/radius.",
                           (min_slots * slot_bytes) >> 20, min_slots, slot_bytes >> 20, free_b >> 20);
+            if (len < 0 || size_t(len) >= sizeof buf)
+                return set_err(err, errlen, VSZ_ENOMEM,
+                               "BM3Dv2: fast_fused: not enough device memory for the accumulator cache. Use "
+                               "fast_fused=False.");
             return set_err(err, errlen, VSZ_ENOMEM, buf);
         }
     }
@@ -684,7 +618,7 @@ int Bm3d::init_device(char *err, size_t errlen) {
 // Fills slot.d_src for the window idx[0..tw) of every clip. With the source cache, frames are
 // uploaded once into cache slots and gathered device-to-device.
 int Bm3d::upload_window(StreamSlot &s, const int *idx, int count, vsz_get_plane_fn get, void *user,
-                        std::vector<int> &held, std::vector<bool> &loaded) {
+                        SlotCache::Lease &lease) {
     if (!scache) {
         // radius 0: stage every plane of the window, one H2D copy per entry
         for (auto &e : entries) {
@@ -707,20 +641,12 @@ int Bm3d::upload_window(StreamSlot &s, const int *idx, int count, vsz_get_plane_
     std::vector<int64_t> keys(nk);
     for (int c = 0; c < n_clips; ++c)
         for (int t = 0; t < count; ++t) keys[c * count + t] = (int64_t(c) << 40) | idx[t];
-    held.assign(nk, -1);
-    std::vector<char> load(nk, 0);
-    {
-        std::vector<int> ih(nk);
-        bool *lp = reinterpret_cast<bool *>(load.data());
-        cache.acquire(keys.data(), nk, ih.data(), lp);
-        held = ih;
-    }
-    loaded.assign(nk, false);
-    for (int k = 0; k < nk; ++k) loaded[k] = load[k] != 0;
+    lease = cache.lease(keys.data(), nk);
+    if (!lease.held()) return vsz_fail(VSZ_ENOMEM, "source cache smaller than one frame window");
     for (int k = 0; k < nk; ++k) {
-        if (!loaded[k]) continue;
+        if (!lease.must_fill(k)) continue;
         const int c = k / count, t = k % count;
-        FrameCache::Slot &slot = cache.slots[held[k]];
+        SlotCache::Buffer &slot = cache.buffer(lease, k);
         // stage through pinned memory, then H2D into the slot
         for (auto &e : entries)
             for (int i = 0; i < e.n_planes; ++i) {
@@ -734,20 +660,18 @@ int Bm3d::upload_window(StreamSlot &s, const int *idx, int count, vsz_get_plane_
         HIPCHK(hipEventRecord(slot.ev, s.stream));
         // the staging buffer is reused by the next frame: wait for this copy
         HIPCHK(hipStreamSynchronize(s.stream));
-        cache.publish(held[k]);
-        loaded[k] = false;  // published: nothing left to abandon
+        lease.filled(k);
     }
-    for (int k = 0; k < nk; ++k) HIPCHK(hipStreamWaitEvent(s.stream, cache.slots[held[k]].ev, 0));
+    for (int k = 0; k < nk; ++k) HIPCHK(hipStreamWaitEvent(s.stream, cache.buffer(lease, k).ev, 0));
     return VSZ_OK;
 }
 
 // Gathers window frames from source-cache slots into slot.d_src (src/bm3d.zig:327-340).
-int gather_from_cache(Bm3d &b, StreamSlot &s, FrameCache &cache, const std::vector<int> &held, int count,
-                      const int *slot_of_t) {
+int gather_from_cache(Bm3d &b, StreamSlot &s, const SlotCache::Lease &src, int count, const int *slot_of_t) {
     for (auto &e : b.entries)
         for (int c = 0; c < b.n_clips; ++c)
             for (int t = 0; t < b.tw; ++t) {
-                const FrameCache::Slot &slot = cache.slots[held[c * count + slot_of_t[t]]];
+                const SlotCache::Buffer &slot = b.cache.buffer(src, c * count + slot_of_t[t]);
                 for (int i = 0; i < e.n_planes; ++i) {
                     const size_t dst_off = e.src_off + ((size_t(c) * e.n_planes + i) * b.tw + t) * e.plane_extent;
                     const size_t coff = e.cache_off + size_t(i) * e.plane_extent;
@@ -798,23 +722,18 @@ int Bm3d::process_normal(StreamSlot &s, int n, vsz_get_plane_fn get, void *user,
     const int nf = vi.num_frames;
     int idx[MAX_TW];
     for (int t = 0; t < tw; ++t) idx[t] = std::min(std::max(n - radius + t, 0), nf - 1);
-    std::vector<int> held;
-    std::vector<bool> loaded;
-    int rc = upload_window(s, idx, tw, get, user, held, loaded);
+    SlotCache::Lease src;
+    int rc = upload_window(s, idx, tw, get, user, src);
     if (rc == VSZ_OK && scache) {
         int slot_of_t[MAX_TW];
         for (int t = 0; t < tw; ++t) slot_of_t[t] = t;
-        rc = gather_from_cache(*this, s, cache, held, tw, slot_of_t);
+        rc = gather_from_cache(*this, s, src, tw, slot_of_t);
     }
     float *out = radius == 0 ? out_base(s) : s.d_out;
     if (rc == VSZ_OK) rc = launch_entries(s, out, radius == 0 ? 0 : 1);
     if (rc == VSZ_OK) rc = download(s, out, radius > 0, dst, dst_stride);
-    if (rc != VSZ_OK) (void)hipStreamSynchronize(s.stream);  // drain before releasing slots
-    if (!held.empty()) {
-        for (size_t k = 0; k < held.size(); ++k)
-            if (k < loaded.size() && loaded[k]) cache.abandon(held[k]);
-        cache.release(held.data(), int(held.size()));
-    }
+    if (rc != VSZ_OK) (void)hipStreamSynchronize(s.stream);  // drain before the buffers are released
+    src.end();
     return rc;
 }
 
@@ -826,46 +745,40 @@ int Bm3d::process_fused(StreamSlot &s, int n, vsz_get_plane_fn get, void *user, 
     const int span = std::min(n + 2 * r, nf - 1) - lo + 1;
     int fidx[2 * MAX_TW];
     for (int j = 0; j < span; ++j) fidx[j] = lo + j;
-    std::vector<int> held;
-    std::vector<bool> loaded;
-    int rc = upload_window(s, fidx, span, get, user, held, loaded);
+    SlotCache::Lease src, stacks;
+    int rc = upload_window(s, fidx, span, get, user, src);
 
     int64_t akeys[MAX_TW];
-    int aidx[MAX_TW];
-    bool aload[MAX_TW];
     int centres[MAX_TW];
-    bool acc_held = false;
     if (rc == VSZ_OK) {
         for (int i = 0; i < tw; ++i) {
             centres[i] = std::min(std::max(n - r + i, 0), nf - 1);
             akeys[i] = centres[i];
         }
-        acc.acquire(akeys, tw, aidx, aload);
-        acc_held = true;
+        stacks = acc.lease(akeys, tw);
+        if (!stacks.held()) rc = vsz_fail(VSZ_ENOMEM, "accumulator cache smaller than one window");
+        // one pass per centre frame whose stack is not resident (src/bm3d.zig:557-605)
         for (int i = 0; i < tw && rc == VSZ_OK; ++i) {
-            if (!aload[i]) continue;
+            if (!stacks.must_fill(i)) continue;
             const int m = centres[i];
             int slot_of_t[MAX_TW];
             for (int t = 0; t < tw; ++t) slot_of_t[t] = std::min(std::max(m - r + t, 0), nf - 1) - lo;
-            rc = gather_from_cache(*this, s, cache, held, span, slot_of_t);
-            if (rc == VSZ_OK) rc = launch_entries(s, static_cast<float *>(acc.slots[aidx[i]].buf), 1);
-            if (rc == VSZ_OK && hipEventRecord(acc.slots[aidx[i]].ev, s.stream) != hipSuccess) rc = VSZ_EDEVICE;
-            if (rc == VSZ_OK) {
-                acc.publish(aidx[i]);
-                aload[i] = false;
-            }
+            rc = gather_from_cache(*this, s, src, span, slot_of_t);
+            if (rc == VSZ_OK) rc = launch_entries(s, static_cast<float *>(acc.buffer(stacks, i).buf), 1);
+            if (rc == VSZ_OK && hipEventRecord(acc.buffer(stacks, i).ev, s.stream) != hipSuccess) rc = VSZ_EDEVICE;
+            if (rc == VSZ_OK) stacks.filled(i);
         }
     }
     if (rc == VSZ_OK) {
         for (int i = 0; i < tw; ++i)
-            if (hipStreamWaitEvent(s.stream, acc.slots[aidx[i]].ev, 0) != hipSuccess) rc = VSZ_EDEVICE;
+            if (hipStreamWaitEvent(s.stream, acc.buffer(stacks, i).ev, 0) != hipSuccess) rc = VSZ_EDEVICE;
         for (auto &e : entries) {
             if (rc) break;
             for (int pi = 0; pi < e.n_planes && rc == VSZ_OK; ++pi) {
                 if (!((e.kp.proc_mask >> pi) & 1)) continue;
                 const float *slots[MAX_TW];
                 for (int i = 0; i < tw; ++i)
-                    slots[i] = static_cast<const float *>(acc.slots[aidx[i]].buf) + e.res_off +
+                    slots[i] = static_cast<const float *>(acc.buffer(stacks, i).buf) + e.res_off +
                                size_t(pi) * tw * 2 * e.plane_extent;
                 rc = vsz_vaggregate_run(e.w, e.h, e.S, r, n, nf, slots,
                                         out_base(s) + e.dst_off + size_t(pi) * e.plane_extent, s.stream);
@@ -873,17 +786,9 @@ int Bm3d::process_fused(StreamSlot &s, int n, vsz_get_plane_fn get, void *user, 
         }
     }
     if (rc == VSZ_OK) rc = download(s, out_base(s), false, dst, dst_stride);
-    if (rc != VSZ_OK) (void)hipStreamSynchronize(s.stream);
-    if (acc_held) {
-        for (int i = 0; i < tw; ++i)
-            if (aload[i]) acc.abandon(aidx[i]);
-        acc.release(aidx, tw);
-    }
-    if (!held.empty()) {
-        for (size_t k = 0; k < held.size(); ++k)
-            if (k < loaded.size() && loaded[k]) cache.abandon(held[k]);
-        cache.release(held.data(), int(held.size()));
-    }
+    if (rc != VSZ_OK) (void)hipStreamSynchronize(s.stream);  // drain before the buffers are released
+    stacks.end();
+    src.end();
     return rc;
 }
 
@@ -1108,7 +1013,8 @@ int create_vagg(const vsz_clip_info *clip, const vsz_clip_info *src, const int *
     const int ratio = clip->height / src->height;
     if (ratio < 6 || (ratio - 2) % 4 != 0 || clip->height % src->height != 0 || clip->width != src->width)
         return fail("VAggregate: \"clip\" is not a BM3D stacked clip of \"src\" (radius >= 1).");
-    auto v = std::make_unique<VAggregate>();
+    std::unique_ptr<VAggregate> v(new (std::nothrow) VAggregate());
+    if (!v) return fail("VAggregate: out of memory.");  // src/bm3d.zig:1419-1421
     v->radius = (ratio - 2) / 4;
     if (v->radius > MAX_RADIUS) return fail("VAggregate: radius too large.");
     for (int i = 0; i < n_planes; ++i) {
