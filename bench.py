@@ -50,6 +50,35 @@ def ssd_candidates(width, height, step, rng):
     return int(nx.sum()) * int(ny.sum())
 
 
+PEAK_BF16_TFLOPS = 2500.0
+# dense FLOPs per pixel of the CNN as inference runs it: 2->32 and 32->1 end layers, four 32->32
+# middle layers, 3x3 taps, 2 FLOPs per multiply-add
+CNN_FLOP_PER_PX = 2 * 9 * (2 * 32 + 4 * 32 * 32 + 32 * 1)
+
+
+def num_blocks(width, height, step):
+    return len(origins(width, step)) * len(origins(height, step))
+
+
+def temporal_candidates(width, height, step, radius, ps_num, ps_range):
+    """Predictive-search candidates of one temporal pass: 2r frames x ps_num windows of
+    (2 ps_range + 1)^2 per reference block, counted unclipped (windows that cross the plane edge
+    scan fewer; an upper bound within a few percent at 1080p)."""
+    return num_blocks(width, height, step) * 2 * radius * ps_num * (2 * ps_range + 1) ** 2
+
+
+def step_roofline(ssd_flop, tf_flop, dt, *, split_bf16=False, mfma_flop=0.0):
+    """T_min = SSD_FLOP / P_fp32 + TF_FLOP / P_tf (+ other MFMA work / P_bf16), against the step
+    time dt: P_tf = P_fp32, or P_bf16 / 3 for split-compensated BF16 transforms."""
+    p_tf = PEAK_BF16_TFLOPS / 3 if split_bf16 else PEAK_FP32_TFLOPS
+    t_min = ssd_flop / (PEAK_FP32_TFLOPS * 1e12) + tf_flop / (p_tf * 1e12) + mfma_flop / (PEAK_BF16_TFLOPS * 1e12)
+    out = {"ssd_gflop": round(ssd_flop / 1e9, 3), "tf_gflop": round(tf_flop / 1e9, 3),
+           "t_min_ms": round(t_min * 1e3, 4), "frac": round(t_min / dt, 4)}
+    if mfma_flop:
+        out["mfma_gflop"] = round(mfma_flop / 1e9, 3)
+    return out
+
+
 def _lib_cnn_ws(n, h, w):
     from vszipcu import _lib
     return _lib.load().vsz_cnn_workspace_size(n, h, w)
@@ -180,7 +209,9 @@ def bench_batch(args):
     torch.cuda.synchronize()
     dt_seq = (time.perf_counter() - t0) / max(1, args.steps // 2)
     tiles = sorted({b.tile(i, 0) for i in range(64)} | {b.tile(i, 1) for i in range(64)})
-    print(json.dumps({"metric": "BM3D batched uneven planes Mpix/s", "value": round(px / dt / 1e6, 3),
+    ssd = sum(192.0 * 2 * ssd_candidates(w, h, 4, 12) for w, h in sizes)
+    tf = sum(128.0 * num_blocks(w, h, 4) * (384 + 576) for w, h in sizes)
+    return {"metric": "BM3D batched uneven planes Mpix/s", "value": round(px / dt / 1e6, 3),
                       "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                       "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "dtype": "f32",
                       "data": "synthetic",
@@ -188,7 +219,8 @@ def bench_batch(args):
                                              "two-pass, bs 4, R 12, extractor 3, one hipGraph per run",
                                  "mpix": round(px / 1e6, 2), "sequential_ms": round(dt_seq * 1e3, 3),
                                  "sequential_mpix_s": round(px / dt_seq / 1e6, 3),
-                                 "setup_s": round(t_setup, 2), "autotuned_tiles": tiles}}), flush=True)
+                                 "setup_s": round(t_setup, 2), "autotuned_tiles": tiles},
+            "step_roofline": step_roofline(ssd, tf, dt)}
 
 
 _WINDOW_IDX = {}
@@ -259,19 +291,21 @@ def bench_temporal(args):
         dt = _timed(lambda: _fused_chain(kp, frames, None, work, acc, out, r, stream), args.steps, args.warmup)
         got = out.cpu().numpy()
         psnr = float(np.mean([synth.psnr(got[i], clean[i]) for i in range(nf)]))
+        ssd = nf * 192.0 * (ssd_candidates(W, H, 8, 9) + temporal_candidates(W, H, 8, r, 2, 4))
+        tf = nf * 128.0 * 384 * num_blocks(W, H, 8)
         res[name] = {"value": round(nf * W * H / dt / 1e6, 3), "ms_per_step": round(dt * 1e3, 3),
-                     "psnr_db": round(psnr, 4)}
+                     "psnr_db": round(psnr, 4), "step_roofline": step_roofline(ssd, tf, dt, split_bf16=prec != "fp32")}
         print(f"bench: temporal {name} {dt * 1e3:.2f} ms / 16 frames, {nf * W * H / dt / 1e6:.1f} Mpix/s, "
               f"PSNR {psnr:.3f} dB", file=sys.stderr, flush=True)
     res["split_bf16_dct"]["psnr_delta_db"] = round(res["split_bf16_dct"]["psnr_db"] - res["plain_fp32"]["psnr_db"], 4)
-    print(json.dumps({"metric": "BM3Dv2 temporal (radius 2) 1080p Gray Mpix/s", "value": res["plain_fp32"]["value"],
+    return {"metric": "BM3Dv2 temporal (radius 2) 1080p Gray Mpix/s", "value": res["plain_fp32"]["value"],
                       "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                       "ms_per_step": res["plain_fp32"]["ms_per_step"], "higher_is_better": True, "dtype": "f32",
                       "data": "synthetic",
                       "config": {"workload": "16 frames 1920x1080 Gray with motion (seed 7), sigma 8, basic pass, "
                                              "radius 2, bs 8, R 9, ps 2/4, extractor 3, fused accumulators + "
                                              "VAggregate, noisy input " f"PSNR {np.mean([synth.psnr(nz[i], clean[i]) for i in range(nf)]):.2f} dB"},
-                      "forms": res}), flush=True)
+            "forms": res}
 
 
 def bench_learned4k(args):
@@ -312,19 +346,25 @@ def bench_learned4k(args):
         dt = _timed(fn, args.steps, args.warmup)
         got = out.cpu().numpy()
         psnr = float(np.mean([synth.psnr(got[i], clean[i]) for i in range(nf)]))
+        ssd_pass = nf * 192.0 * (ssd_candidates(W4, H4, 8, 9) + temporal_candidates(W4, H4, 8, r, 2, 4))
+        nb = num_blocks(W4, H4, 8)
+        if name == "plain":
+            roof = step_roofline(2 * ssd_pass, nf * 128.0 * nb * (384 + 576), dt)
+        else:
+            roof = step_roofline(ssd_pass, nf * 128.0 * nb * 576, dt, mfma_flop=nf * CNN_FLOP_PER_PX * W4 * H4)
         res[name] = {"value": round(nf * W4 * H4 / dt / 1e6, 3), "ms_per_step": round(dt * 1e3, 3),
-                     "psnr_db": round(psnr, 4)}
+                     "psnr_db": round(psnr, 4), "step_roofline": roof}
         print(f"bench: 4K {name} {dt * 1e3:.2f} ms / {nf} frames, {nf * W4 * H4 / dt / 1e6:.1f} Mpix/s, "
               f"PSNR {psnr:.3f} dB", file=sys.stderr, flush=True)
     res["learned_cnn_basic"]["psnr_delta_db"] = round(res["learned_cnn_basic"]["psnr_db"] - res["plain"]["psnr_db"], 4)
-    print(json.dumps({"metric": "BM3Dv2 4K Gray Wiener radius 1 Mpix/s (plain vs learned basic pass)",
+    return {"metric": "BM3Dv2 4K Gray Wiener radius 1 Mpix/s (plain vs learned basic pass)",
                       "value": res["plain"]["value"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
                       "warmup": args.warmup, "ms_per_step": res["plain"]["ms_per_step"], "higher_is_better": True,
                       "dtype": "f32", "data": "synthetic",
                       "config": {"workload": "8 frames 3840x2160 Gray with motion (seed 8), sigma 10, basic + Wiener "
                                              "at radius 1, bs 8, R 9, ps 2/4, extractor 3, fused accumulators + "
                                              "VAggregate; learned form: CNN basic estimate per frame"},
-                      "forms": res}), flush=True)
+            "forms": res}
 
 
 def main():
@@ -335,6 +375,8 @@ def main():
     ap.add_argument("--cpu-strip-rows", type=int, default=136, help="rows of the single-thread CPU sample")
     ap.add_argument("--no-end-to-end", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the temporal16 / batch64 / learned4k lines of the default run")
     ap.add_argument("--workload", choices=["frame1080p", "batch64", "temporal16", "learned4k"], default="frame1080p",
                     help="batch64: 64 uneven Gray planes in one batched plan; temporal16: 16 frames 1080p Gray at "
                          "radius 2 through the fused chain; learned4k: 4K Gray Wiener at radius 1 after the plain or "
@@ -354,12 +396,10 @@ def main():
     assert torch.cuda.is_available(), "bench needs a GPU"
     if args.profile_target != "step":
         return profile_target(args)
-    if args.workload == "batch64":
-        return bench_batch(args)
-    if args.workload == "temporal16":
-        return bench_temporal(args)
-    if args.workload == "learned4k":
-        return bench_learned4k(args)
+    side = {"batch64": bench_batch, "temporal16": bench_temporal, "learned4k": bench_learned4k}
+    if args.workload in side:
+        print(json.dumps(side[args.workload](args)), flush=True)
+        return
     print(f"bench: workload 1920x1080 YUV444 two-pass BM3D, steps={args.steps} warmup={args.warmup}", file=sys.stderr,
           flush=True)
     clean, nz = synth.yuv444_frame(W, H, SIGMA, seed=2)
@@ -386,6 +426,8 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     mpix = W * H / dt / 1e6
     print(f"bench: {dt * 1e3:.3f} ms/step, {mpix:.2f} Mpix/s", file=sys.stderr, flush=True)
+    t_basic = _timed(lambda: device.run(kp_b, d_noisy, None, work_b, d_basic.view(3, H, W), 0, stream),
+                     args.steps, args.warmup)
 
     # per-kernel timing with HIP events on the launch stream: the matching kernels of both
     # passes (re-launched alone on the same inputs) and the remainder of each pass
@@ -419,6 +461,7 @@ def main():
 
     out = d_final.cpu().numpy()
     psnr_y = synth.psnr(out[0], clean[0])
+    psnr_planes = [synth.psnr(out[p], clean[p]) for p in range(3)]
 
     # the same two passes with the DCT stages as compensated split-BF16 MFMA products
     kp_b16 = device.kparams(W, H, sigma=device.scaled_sigma(SIGMA, False), precision="split-bf16", **PARAMS)
@@ -466,9 +509,12 @@ def main():
                 step_learned(kpf)
             torch.cuda.synchronize()
             dtl = (time.perf_counter() - t0) / args.steps
-            pl = synth.psnr(d_final.cpu().numpy()[0], clean[0])
+            fl = d_final.cpu().numpy()
+            pl = synth.psnr(fl[0], clean[0])
+            deltas = [round(synth.psnr(fl[p], clean[p]) - psnr_planes[p], 4) for p in range(3)]
             learned[name] = {"value": round(W * H / dtl / 1e6, 3), "ms_per_step": round(dtl * 1e3, 4),
-                             "psnr_y_db": round(pl, 4), "psnr_delta_db": round(pl - psnr_y, 4)}
+                             "psnr_y_db": round(pl, 4), "psnr_delta_db": round(pl - psnr_y, 4),
+                             "psnr_delta_db_yuv": deltas}
             print(f"bench: {name} {dtl * 1e3:.3f} ms/step, {W * H / dtl / 1e6:.2f} Mpix/s, PSNR {pl:.4f} dB "
                   f"(delta {pl - psnr_y:+.4f})", file=sys.stderr, flush=True)
         # the CNN alone, per step (3 planes)
@@ -480,10 +526,27 @@ def main():
         ev1.record(stream)
         torch.cuda.synchronize()
         t_cnn = ev0.elapsed_time(ev1) / args.steps * 1e-3
-        cnn_flop = 6 * 2.0 * 9 * 32 * 32 * W * H * 3   # padded 32->32 layers, as computed
+        cnn_flop = float(CNN_FLOP_PER_PX) * W * H * 3   # the FLOPs inference executes (end layers unpadded)
         learned["cnn_ms"] = round(t_cnn * 1e3, 4)
         learned["cnn_tflops"] = round(cnn_flop / t_cnn / 1e12, 2)
-        learned["weights"] = {k: model.meta.get(k) for k in ("pairs", "patch", "epochs", "steps", "train_s")}
+        learned["weights"] = {k: model.meta.get(k) for k in ("pairs", "patch", "epochs", "steps", "train_s",
+                                                             "chroma_frac", "log_sigma")}
+        # the judge: the learned basic pass runs only if every plane's (kind, sigma) has a measured
+        # PSNR delta within the bound; otherwise the plain basic pass is the fallback for the frame
+        table = cnn.load_judge()
+        kinds = ["luma", "chroma", "chroma"]
+        allowed = [cnn.learned_allowed(SIGMA[p], kinds[p], table) for p in range(3)]
+        use_learned = all(allowed)
+        chosen = learned["learned_bf16"] if use_learned else {"value": round(mpix, 3), "ms_per_step": round(dt * 1e3, 4),
+                                                              "psnr_delta_db_yuv": [0.0, 0.0, 0.0]}
+        learned["judged"] = {"basic_pass": "learned" if use_learned else "plain (fallback)",
+                             "planes_allowed": allowed, "bound_db": cnn.JUDGE_BOUND_DB, **chosen}
+        # what even a free basic pass could give: the Wiener pass alone bounds the learned form
+        t_wiener = dt - t_basic
+        learned["speedup_bound"] = {"wiener_pass_ms": round(t_wiener * 1e3, 4),
+                                    "max_speedup_vs_plain": round(dt / t_wiener, 3)}
+        print(f"bench: judge -> {learned['judged']['basic_pass']} (planes allowed {allowed}); a free basic pass "
+              f"would give at most {dt / t_wiener:.2f}x", file=sys.stderr, flush=True)
 
     nbx, nby = device.num_blocks(kp_b)
     tf_flop = 128.0 * nbx * nby * 3 * (384 + 576)
@@ -509,7 +572,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "BM3Dv2 two-pass (basic + Wiener ref=basic), chroma, 1920x1080 YUV444PS",
                    "sigma": SIGMA, "path": "plain FP32", **{k: v for k, v in PARAMS.items()},
-                   "psnr_y_db": round(psnr_y, 3), "match_ms_per_step": round(t_match * 1e3, 4)},
+                   "psnr_y_db": round(psnr_y, 3), "psnr_yuv_db": [round(v, 3) for v in psnr_planes],
+                   "match_ms_per_step": round(t_match * 1e3, 4), "basic_pass_ms": round(t_basic * 1e3, 4)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": "k_match_lds4 (SSD block matching, both passes)"},
@@ -521,6 +585,19 @@ def main():
                                      "note": "DCT stages as compensated split-BF16 MFMA, FP32 accumulation; "
                                              "matching FP32"}},
     }
+    if not args.no_workloads:
+        # the temporal, batched and 4K workloads, fewer steps each; not the headline `value`
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = max(2, min(args.steps, 8)), max(1, min(args.warmup, 2))
+        result["workloads"] = {}
+        for name, fn in (("temporal16", bench_temporal), ("batch64", bench_batch), ("learned4k", bench_learned4k)):
+            print(f"bench: workload {name} ...", file=sys.stderr, flush=True)
+            r = fn(sub)
+            result["workloads"][name] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config")
+                                         if k in r}
+            for k in ("forms", "step_roofline"):
+                if k in r:
+                    result["workloads"][name][k] = r[k]
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(nz, args.cpu_strip_rows)

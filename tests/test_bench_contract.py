@@ -43,3 +43,19 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert p.returncode == 2
     assert "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_workload_work_counts():
+    """Blocks and temporal candidates of the temporal (1080p, bs 8, R 9, r 2, ps 2/4) and 4K
+    (bs 8, R 9, r 1) workloads; the CNN's executed FLOPs per pixel (2->32, 4 x 32->32, 32->1)."""
+    assert bench.num_blocks(1920, 1080, 8) == 240 * 135
+    assert bench.num_blocks(3840, 2160, 8) == 480 * 270
+    assert bench.temporal_candidates(1920, 1080, 8, 2, 2, 4) == 32400 * 648
+    assert bench.temporal_candidates(3840, 2160, 8, 1, 2, 4) == 129600 * 324
+    ssd = 192 * (bench.ssd_candidates(1920, 1080, 8, 9) + bench.temporal_candidates(1920, 1080, 8, 2, 2, 4))
+    assert abs(ssd / 1e9 - 6.28) < 0.05
+    assert bench.CNN_FLOP_PER_PX == 2 * 9 * (2 * 32 + 4 * 32 * 32 + 32)
+    r = bench.step_roofline(157.3e12 * 1e-3, 0.0, 2e-3)
+    assert abs(r["t_min_ms"] - 1.0) < 1e-9 and abs(r["frac"] - 0.5) < 1e-9
+    r = bench.step_roofline(0.0, 2500e12 / 3 * 1e-3, 1e-3, split_bf16=True)
+    assert abs(r["frac"] - 1.0) < 1e-9

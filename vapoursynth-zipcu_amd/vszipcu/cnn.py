@@ -182,10 +182,12 @@ BASIC_PARAMS = dict(block_step=4, bm_range=16, extractor_exp=3)
 
 
 def make_pairs(count: int, patch: int = 128, frame: int = 1024, draws_per_frame: int = 8, seed: int = 0,
-               sigma_range=(2.0, 25.0)):
+               sigma_range=(2.0, 25.0), chroma_frac: float = 0.0, log_sigma: bool = False):
     """(noisy, basic, sigma/255) patches: synthetic clean frames, Gaussian noise with sigma drawn
-    from sigma_range (per frame draw), BM3D basic pass of the engine on the whole frame, then the
-    frame cut into patch x patch tiles. Returns device tensors [count, patch, patch] x 2, [count]."""
+    from sigma_range (per frame draw; log-uniform with log_sigma), BM3D basic pass of the engine on
+    the whole frame, then the frame cut into patch x patch tiles. A fraction chroma_frac of the
+    clean frames comes from the low-frequency chroma generator ([-0.5, 0.5]) instead of the luma
+    one. Returns device tensors [count, patch, patch] x 2, [count]."""
     import torch
     per = (frame // patch) ** 2
     draws = (count + per - 1) // per
@@ -196,8 +198,12 @@ def make_pairs(count: int, patch: int = 128, frame: int = 1024, draws_per_frame:
     clean = None
     for d in range(draws):
         if d % draws_per_frame == 0:
-            clean = torch.from_numpy(synth.clean_luma(frame, frame, seed * 1000 + d)).cuda()
-        s255 = float(rng.uniform(*sigma_range))
+            gen_fn = synth.clean_chroma if rng.random() < chroma_frac else synth.clean_luma
+            clean = torch.from_numpy(gen_fn(frame, frame, seed * 1000 + d)).cuda()
+        if log_sigma:
+            s255 = float(np.exp(rng.uniform(np.log(sigma_range[0]), np.log(sigma_range[1]))))
+        else:
+            s255 = float(rng.uniform(*sigma_range))
         nz = clean + torch.randn(frame, frame, generator=gen, device="cuda") * (s255 / 255.0)
         kp = device.kparams(frame, frame, sigma=device.scaled_sigma([s255] * 3, False), **BASIC_PARAMS)
         if kp_work is None:
@@ -212,11 +218,12 @@ def make_pairs(count: int, patch: int = 128, frame: int = 1024, draws_per_frame:
             torch.cat(sig_p)[:count].contiguous())
 
 
-def train(count=20480, patch=128, batch=32, epochs=8, lr=1e-3, seed=0, log=print):
+def train(count=20480, patch=128, batch=32, epochs=8, lr=1e-3, seed=0, log=print, chroma_frac=0.0,
+          log_sigma=False):
     """Generates the pairs, trains with Adam and a cosine learning-rate decay, returns the model."""
     import torch
     t0 = time.perf_counter()
-    noisy, basic, sig = make_pairs(count, patch, seed=seed)
+    noisy, basic, sig = make_pairs(count, patch, seed=seed, chroma_frac=chroma_frac, log_sigma=log_sigma)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     m = Model.init(seed)
@@ -238,9 +245,79 @@ def train(count=20480, patch=128, batch=32, epochs=8, lr=1e-3, seed=0, log=print
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     m.meta = {"pairs": count, "patch": patch, "batch": batch, "epochs": epochs, "steps": steps, "lr": lr,
-              "seed": seed, "basic_params": BASIC_PARAMS, "sigma255": [2.0, 25.0], "data_s": round(t_data, 2),
+              "seed": seed, "basic_params": BASIC_PARAMS, "sigma255": [2.0, 25.0], "chroma_frac": chroma_frac,
+              "log_sigma": log_sigma, "data_s": round(t_data, 2),
               "train_s": round(t_train, 2), "loss_history": hist}
     return m
+
+
+# ---- the judge: where the learned basic pass may replace the plain one --------------------------
+JUDGE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "cnn_judge.json")
+JUDGE_BOUND_DB = 0.05
+JUDGE_PARAMS = dict(block_step=4, bm_range=16, extractor_exp=3)
+
+
+def _two_pass_psnr(noisy_np, clean_np, s255, model, work_cache):
+    """(PSNR of plain basic + Wiener, PSNR of learned basic + Wiener) of one Gray plane."""
+    import torch
+    h, w = noisy_np.shape
+    nz = torch.from_numpy(noisy_np).cuda()
+    kb = device.kparams(w, h, sigma=device.scaled_sigma([s255] * 3, False), **JUDGE_PARAMS)
+    kf = device.kparams(w, h, sigma=device.scaled_sigma([s255] * 3, True), final=True, **JUDGE_PARAMS)
+    need = max(device.workspace_size(kb), device.workspace_size(kf))
+    if work_cache.get("n", 0) < need:
+        work_cache["t"] = torch.zeros(need, dtype=torch.uint8, device="cuda")
+        work_cache["n"] = need
+    work = work_cache["t"]
+    basic = torch.zeros((1, h, w), dtype=torch.float32, device="cuda")
+    out = torch.zeros((1, h, w), dtype=torch.float32, device="cuda")
+    device.run(kb, nz[None, None], None, work, basic, 0)
+    device.run(kf, basic[:, None], nz[None, None], work, out, 0)
+    plain = synth.psnr(out[0].cpu().numpy(), clean_np)
+    model.run(nz[None], s255, out=basic)
+    device.run(kf, basic[:, None], nz[None, None], work, out, 0)
+    learned = synth.psnr(out[0].cpu().numpy(), clean_np)
+    return plain, learned
+
+
+def measure_judge(model=None, sigmas=(2.0, 3.0, 5.0, 8.0, 12.0, 16.0, 20.0, 25.0), size=768, seeds=(11, 12, 13)):
+    """PSNR delta (learned basic + Wiener minus plain basic + Wiener, against the clean plane) per
+    sigma and plane kind, averaged over seeded synthetic planes: "luma" from the luma generator,
+    "chroma" from the low-frequency chroma generator ([-0.5, 0.5]). Runs on the GPU."""
+    model = model or Model.load()
+    cache = {}
+    table = {"bound_db": JUDGE_BOUND_DB, "params": JUDGE_PARAMS, "size": size, "seeds": list(seeds), "rows": []}
+    for kind in ("luma", "chroma"):
+        for s255 in sigmas:
+            d = []
+            for sd in seeds:
+                clean = synth.clean_luma(size, size, sd) if kind == "luma" else synth.clean_chroma(size, size, sd)
+                nz = synth.noisy(clean, s255, sd + 500)
+                pp, pl = _two_pass_psnr(nz, clean, s255, model, cache)
+                d.append(pl - pp)
+            table["rows"].append({"kind": kind, "sigma": s255, "delta_db": round(float(np.mean(d)), 4)})
+    return table
+
+
+def load_judge(path: str = JUDGE_PATH) -> Optional[dict]:
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def learned_allowed(sigma255: float, kind: str, table: Optional[dict]) -> bool:
+    """The judge's rule: the learned basic pass replaces the plain one for a plane of this kind
+    and sigma only if the measured PSNR delta at both measured sigmas around it (or at that sigma)
+    is within the bound; outside the measured range, or without a table, the plain pass runs."""
+    if not table:
+        return False
+    rows = sorted((r["sigma"], r["delta_db"]) for r in table["rows"] if r["kind"] == kind)
+    if not rows or sigma255 < rows[0][0] or sigma255 > rows[-1][0]:
+        return False
+    lo = max(r for r in rows if r[0] <= sigma255)
+    hi = min(r for r in rows if r[0] >= sigma255)
+    return lo[1] >= -table["bound_db"] and hi[1] >= -table["bound_db"]
 
 
 def main(argv=None):
@@ -254,8 +331,20 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--chroma-frac", type=float, default=0.0, help="share of chroma-generator training frames")
+    ap.add_argument("--log-sigma", action="store_true", help="draw training sigmas log-uniformly")
+    ap.add_argument("--judge", action="store_true",
+                    help="measure the PSNR-delta table of the saved weights into vszipcu/data/cnn_judge.json")
     a = ap.parse_args(argv)
-    m = train(a.pairs, a.patch, a.batch, a.epochs, a.lr, a.seed, log=lambda s: print(s, flush=True))
+    if a.judge:
+        t = measure_judge(Model.load(a.out))
+        with open(JUDGE_PATH, "w") as f:
+            json.dump(t, f, indent=1)
+        for r in t["rows"]:
+            print(f"cnn judge: {r['kind']:6s} sigma {r['sigma']:5.1f}: delta {r['delta_db']:+.4f} dB", flush=True)
+        return
+    m = train(a.pairs, a.patch, a.batch, a.epochs, a.lr, a.seed, log=lambda s: print(s, flush=True),
+              chroma_frac=a.chroma_frac, log_sigma=a.log_sigma)
     m.save(a.out)
     print(f"cnn: saved {a.out} ({m.meta['train_s']} s training, {m.meta['data_s']} s data)", flush=True)
 
