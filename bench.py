@@ -576,7 +576,10 @@ def main():
                    "match_ms_per_step": round(t_match * 1e3, 4), "basic_pass_ms": round(t_basic * 1e3, 4)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "k_match_lds4 (SSD block matching, both passes)"},
+                     "kernel": "k_match_lds4 (SSD block matching, both passes)",
+                     "work": "dense-equivalent SSD FLOP, 192 per candidate block; the kernel executes about half "
+                             "(each lane takes half of its distance tree from its neighbour) and issues VALU work "
+                             "on every SIMD cycle (profiles/k_match_lds4_1080p_valu_counters.md)"},
         "step_roofline": {"ssd_gflop": round(flop / 1e9, 3), "tf_gflop": round(tf_flop / 1e9, 3),
                           "t_min_ms": round(t_min * 1e3, 4), "frac": round(t_min / dt, 4)},
         "end_to_end_mpix_s": e2e,

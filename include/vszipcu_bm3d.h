@@ -135,7 +135,9 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
                           vsz_bm3d_batch **out);
 size_t vsz_bm3d_batch_workspace_size(const vsz_bm3d_batch *b);
 /* d_noisy[i]: [height][stride] input, d_out[i]: [height][stride] result of plane i. The first
- * run tunes (when asked) and captures the graph; a run with other pointers re-captures. */
+ * run tunes (when asked) and captures the graph; a run with other pointers re-captures, after
+ * waiting for the previous run's replay (which may be on another stream) to finish. A batch object
+ * serves one call at a time: calls from several threads must be serialised by the caller. */
 int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *const *d_out, void *d_work,
                        void *stream);
 /* Filter tile (in reference blocks) chosen for plane i, pass 0 (basic) or 1 (final). */

@@ -509,6 +509,7 @@ struct OracleParams {
     int t2d, t1d;    // 0 dct, 1 haar, 2 wht, 3 bior1.5
     int threads;     // 0 = all hardware threads
     int by_begin, by_end;  // block rows to process, [by_begin, by_end); by_end 0 = all rows
+    int float_sums;        // 1: FP32 sums in reference-block raster order even with a non-zero extractor
 };
 
 }  // extern "C"
@@ -598,7 +599,7 @@ int run_bm3d(const OracleParams &p, const T *src, T *res, int32_t *matches_out, 
     const long CLIP = long(P) * PLANE;          // one clip: [P][TW][H][S]
     const int threads = num_threads(p.threads);
     const T sig[3] = {T(p.sigma[0]), T(p.sigma[1]), T(p.sigma[2])};
-    const bool exact = std::is_same<T, float>::value && exact_path(p);
+    const bool exact = std::is_same<T, float>::value && exact_path(p) && !p.float_sums;
     const T E = T(p.extractor);
 
     const long res_elems = long(P) * TW * 2 * TS;

@@ -29,6 +29,7 @@ class OracleParams(C.Structure):
         ("metric", C.c_int), ("t2d", C.c_int), ("t1d", C.c_int),
         ("threads", C.c_int),
         ("by_begin", C.c_int), ("by_end", C.c_int),
+        ("float_sums", C.c_int),
     ]
 
 
@@ -75,7 +76,7 @@ def _ptr(a: np.ndarray, ctype):
 
 def make_params(width, height, stride=None, *, block_step=8, bm_range=9, radius=0, ps_num=2,
                 ps_range=4, chroma=False, final=False, sigma=(0.0, 0.0, 0.0), extractor_exp=0,
-                metric="ssd", t2d="dct", t1d="dct", threads=0, block_rows=None) -> OracleParams:
+                metric="ssd", t2d="dct", t1d="dct", threads=0, block_rows=None, float_sums=False) -> OracleParams:
     """Kernel-level parameters. `sigma` holds the SCALED thresholds (sigma * 0x3f021bb6 for
     the basic pass, * 0x3e40c0c1 for the final pass), as the device code receives them."""
     p = OracleParams()
@@ -94,6 +95,7 @@ def make_params(width, height, stride=None, *, block_step=8, bm_range=9, radius=
     p.threads = threads
     if block_rows is not None:
         p.by_begin, p.by_end = block_rows
+    p.float_sums = int(bool(float_sums))
     return p
 
 
