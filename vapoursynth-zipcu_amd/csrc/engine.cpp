@@ -620,7 +620,8 @@ int Bm3d::init_device(char *err, size_t errlen) {
 int Bm3d::upload_window(StreamSlot &s, const int *idx, int count, vsz_get_plane_fn get, void *user,
                         SlotCache::Lease &lease) {
     if (!scache) {
-        // radius 0: stage every plane of the window, one H2D copy per entry
+        // radius 0: stage every plane of the window; each plane's H2D copy is queued as soon as
+        // it is staged, so the DMA of one plane overlaps the host copy of the next
         for (auto &e : entries) {
             for (int c = 0; c < n_clips; ++c)
                 for (int i = 0; i < e.n_planes; ++i)
@@ -630,9 +631,9 @@ int Bm3d::upload_window(StreamSlot &s, const int *idx, int count, vsz_get_plane_
                         if (!src) return vsz_fail(VSZ_EFRAME, "frame source returned no data");
                         const size_t off = e.src_off + ((size_t(c) * e.n_planes + i) * tw + t) * e.plane_extent;
                         copy_plane(s.h_stage + off, ptrdiff_t(e.S) * 4, src, st, size_t(e.w) * 4, e.h);
+                        HIPCHK(hipMemcpyAsync(s.d_src + off, s.h_stage + off, e.plane_extent * 4, hipMemcpyHostToDevice,
+                                              s.stream));
                     }
-            HIPCHK(hipMemcpyAsync(s.d_src + e.src_off, s.h_stage + e.src_off, e.src_elems * 4, hipMemcpyHostToDevice,
-                                  s.stream));
         }
         return VSZ_OK;
     }
