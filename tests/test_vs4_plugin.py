@@ -84,6 +84,20 @@ def test_create_errors_reach_the_output_map(vs, kwargs, msg):
         src.free()
 
 
+@pytest.mark.parametrize("kwargs", [dict(block_step=9), dict(radius=17, sigma=3), dict(chroma=1)])
+def test_bm3dv2_reports_the_invoked_bm3d_error(vs, kwargs):
+    """The composed BM3Dv2 returns the error map of the BM3D it invokes (src/bm3d.zig:1469-1472)."""
+    src = vs.source(_frames(32, 24, 2), GRAYS)
+    try:
+        with pytest.raises(host.HostError) as e1:
+            vs.invoke("BM3D", clip=src, **kwargs)
+        with pytest.raises(host.HostError) as e2:
+            vs.invoke("BM3Dv2", clip=src, **kwargs)
+        assert str(e2.value) == str(e1.value) and str(e1.value).startswith("BM3D: ")
+    finally:
+        src.free()
+
+
 def test_format_error_and_required_clip(vs):
     src = vs.source([[np.zeros((24, 32), np.float32)]], GRAY16)
     try:
@@ -185,12 +199,32 @@ def test_bm3dv2_temporal_two_pass_matches_mirror(vs, torch_gpu, fused):
     mb = core.vszipcu.BM3Dv2(mclip, **kw)
     mf = core.vszipcu.BM3Dv2(mclip, ref=mb, **kw)
     try:
-        assert final.name == "BM3Dv2" and final.deps() == [0, 0]
+        if fused:   # the fused chain is one filter over clip and ref
+            assert final.name == "BM3Dv2" and final.deps() == [0, 0]
+        else:       # composed: VAggregate (General over BM3D's stack, StrictSpatial over src)
+            assert final.name == "VAggregate" and final.deps() == [0, 2]
         for n in range(nf):
             got, _ = final.frame(n)
             assert np.array_equal(got[0], _mirror_frames(mf, n)[0])
     finally:
         for x in (final, basic, src):
+            x.free()
+
+
+@pytest.mark.gpu
+def test_bm3dv2_radius0_is_the_registered_bm3d(vs, torch_gpu):
+    """BM3Dv2 at radius 0 returns the node the registered BM3D creates (src/bm3d.zig:1474-1479)."""
+    W, H = 48, 40
+    _, nz = synth.gray_frame(W, H, 8.0, seed=34)
+    src = vs.source([[nz]], GRAYS)
+    kw = dict(sigma=8, block_step=4, bm_range=6, extractor_exp=3)
+    v2 = vs.invoke("BM3Dv2", clip=src, **kw)
+    v1 = vs.invoke("BM3D", clip=src, **kw)
+    try:
+        assert v2.name == "BM3D" and v2.deps() == [2]
+        assert np.array_equal(v2.frame(0)[0][0], v1.frame(0)[0][0])
+    finally:
+        for x in (v2, v1, src):
             x.free()
 
 

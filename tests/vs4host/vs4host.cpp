@@ -94,6 +94,7 @@ struct VSFrameContext {
 };
 
 struct VSCore {
+    VSPlugin *plugin = nullptr;  // the loaded plugin (getPluginByID)
     std::string plugin_id, plugin_ns, plugin_name;
     int plugin_version = 0, plugin_api = 0;
     struct Fn {
@@ -322,6 +323,17 @@ int mapConsumeNode(VSMap *m, const char *k, VSNode *n, int append) {
     slot(m, k, ptVideoNode, append).n.push_back(n);
     return 0;
 }
+extern "C" VSMap *h_invoke(VSCore *core, const char *name, VSMap *in);
+VSPlugin *getPluginByID(const char *id, VSCore *core) {
+    return core && core->plugin && core->plugin_id == id ? core->plugin : nullptr;
+}
+// A plugin invoking one of its own functions: the same checks as h_invoke, on a copy of the
+// caller's arguments (VapourSynth does not modify them).
+VSMap *invoke(VSPlugin *pl, const char *name, const VSMap *args) {
+    VSMap copy;
+    copy_map(*args, copy);
+    return h_invoke(pl->core, name, &copy);
+}
 int getAPIVersion() { return VSZ_VS_API_VERSION; }
 void logMessage(int type, const char *msg, VSCore *) { std::fprintf(stderr, "vs4host log %d: %s\n", type, msg); }
 
@@ -373,6 +385,8 @@ VSAPI make_api() {
     a.mapConsumeNode = mapConsumeNode;
     a.getAPIVersion = getAPIVersion;
     a.logMessage = logMessage;
+    a.getPluginByID = getPluginByID;
+    a.invoke = invoke;
     return a;
 }
 
@@ -502,8 +516,8 @@ int h_load_plugin(VSCore *core, const char *path, char *err, int errlen) {
         return -2;
     }
     static const VSPLUGINAPI papi = {p_getAPIVersion, p_configPlugin, p_registerFunction};
-    VSPlugin pl{core};
-    init(&pl, &papi);
+    if (!core->plugin) core->plugin = new VSPlugin{core};
+    init(core->plugin, &papi);
     return int(core->fns.size());
 }
 

@@ -140,9 +140,12 @@ struct VSAPI {
     int (*mapConsumeNode)(VSMap *map, const char *key, VSNode *node, int append);
     VSApiSlot mapGetFrame, mapSetFrame, mapConsumeFrame, mapGetFunction, mapSetFunction, mapConsumeFunction;
     /* plugins */
-    VSApiSlot registerFunction, getPluginByID, getPluginByNamespace, getNextPlugin, getPluginName, getPluginID,
-        getPluginNamespace, getNextPluginFunction, getPluginFunctionByName, getPluginFunctionName,
-        getPluginFunctionArguments, getPluginFunctionReturnType, getPluginPath, getPluginVersion, invoke;
+    VSApiSlot registerFunction;
+    VSPlugin *(*getPluginByID)(const char *identifier, VSCore *core);
+    VSApiSlot getPluginByNamespace, getNextPlugin, getPluginName, getPluginID, getPluginNamespace,
+        getNextPluginFunction, getPluginFunctionByName, getPluginFunctionName, getPluginFunctionArguments,
+        getPluginFunctionReturnType, getPluginPath, getPluginVersion;
+    VSMap *(*invoke)(VSPlugin *plugin, const char *name, const VSMap *args);
     /* core */
     VSApiSlot createCore, freeCore, setMaxCacheSize, setThreadCount, getCoreInfo;
     int (*getAPIVersion)(void);

@@ -2,6 +2,7 @@
 // of vszipcu_bm3d.h. This is the part of src/plugin.zig and src/bm3d.zig that talks to
 // VapourSynth: argument maps in, filter nodes out, frames requested in arInitial and handed to
 // the engine in arAllFramesReady.
+#include <cfloat>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -156,8 +157,77 @@ void free_instance(void *inst, VSCore * /*core*/, const VSAPI *api) {
     delete d;
 }
 
+// BM3Dv2 as the reference builds it (src/bm3d.zig:1432-1501): the input itself when no plane is
+// processed; otherwise the registered BM3D, followed at radius > 0 by the registered VAggregate over
+// BM3D's stacked output, as two nodes of the graph. Returns false for fast_fused at radius > 0,
+// which is one filter of this plugin (the fused accumulator chain).
+bool compose_v2(const VSMap *in, VSMap *out, VSCore *core, const VSAPI *api) {
+    int err = 0;
+    bool proc[3] = {true, true, true};
+    const int ns = count(api, in, "sigma");
+    for (int i = 0; i < 3; ++i) {
+        if (i < ns) proc[i] = !(float(api->mapGetFloat(in, "sigma", i, &err)) < FLT_EPSILON);
+        else if (i > 0) proc[i] = proc[i - 1];
+    }
+    VSNode *src = api->mapGetNode(in, "clip", 0, &err);
+    if (!src) {
+        api->mapSetError(out, "BM3D: missing clip.");
+        return true;
+    }
+    const int np = api->getVideoInfo(src)->format.numPlanes;
+    bool skip = true;
+    for (int i = 0; i < np && i < 3; ++i)
+        if (proc[i]) skip = false;
+    if (skip) {
+        api->mapConsumeNode(out, "clip", src, maReplace);
+        return true;
+    }
+    VSPlugin *plugin = api->getPluginByID("com.julek.vszipcu", core);
+    if (!plugin) {
+        api->freeNode(src);
+        api->mapSetError(out, "BM3Dv2: could not find the vszipcu plugin.");
+        return true;
+    }
+    int radius = 0, fused = 0;
+    int_scalar(api, in, "radius", &radius);
+    int_scalar(api, in, "fast_fused", &fused);
+    if (radius > 0 && fused != 0) {
+        api->freeNode(src);
+        return false;
+    }
+    VSMap *bret = api->invoke(plugin, "BM3D", in);
+    if (const char *e = api->mapGetError(bret)) {
+        api->mapSetError(out, e);
+        api->freeMap(bret);
+        api->freeNode(src);
+        return true;
+    }
+    VSNode *bnode = api->mapGetNode(bret, "clip", 0, &err);
+    api->freeMap(bret);
+    if (radius == 0) {
+        api->mapConsumeNode(out, "clip", bnode, maReplace);
+        api->freeNode(src);
+        return true;
+    }
+    VSMap *aargs = api->createMap();
+    api->mapConsumeNode(aargs, "clip", bnode, maReplace);
+    api->mapConsumeNode(aargs, "src", src, maReplace);
+    for (int i = 0; i < 3; ++i)
+        if (proc[i]) api->mapSetInt(aargs, "planes", i, maAppend);
+    VSMap *aret = api->invoke(plugin, "VAggregate", aargs);
+    api->freeMap(aargs);
+    if (const char *e = api->mapGetError(aret)) {
+        api->mapSetError(out, e);
+    } else {
+        api->mapConsumeNode(out, "clip", api->mapGetNode(aret, "clip", 0, &err), maReplace);
+    }
+    api->freeMap(aret);
+    return true;
+}
+
 void create(const VSMap *in, VSMap *out, void *user, VSCore *core, const VSAPI *api) {
     const int kind = int(reinterpret_cast<intptr_t>(user));
+    if (kind == kBm3dV2 && compose_v2(in, out, core, api)) return;
     int err = 0;
     Instance *d = new (std::nothrow) Instance;
     if (!d) {
