@@ -1056,11 +1056,23 @@ __device__ __forceinline__ unsigned pk_bf16(float a, float b) {
     return __builtin_bit_cast(unsigned, __builtin_convertvector((vsz_f2){a, b}, vsz_b2));
 }
 
-// hi = bf16(x), lo = bf16(x - hi) for a pair; hi is widened back by a shift or a mask (exact), so
-// x - hi is exact and the split costs three instructions per value.
+// A 32-bit pattern in a VGPR, so that an instruction reads it as a register operand: the
+// compiler would otherwise encode a packed-bf16 constant such as (-1, 0) as an inline constant.
+__device__ __forceinline__ unsigned vgpr_bits(unsigned bits) {
+    unsigned v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "i"(bits));
+    return v;
+}
+
+// hi = bf16(x), lo = bf16(x - hi) for a pair: two instructions per value. The residual x - hi
+// comes from one bf16 dot product each, x + (-1)*hi[k] + 0*hi[1-k]: the product is exact and x - hi
+// (at most 16 significant bits) is representable, so it is exact before lo rounds it to bf16.
 __device__ __forceinline__ void split2(float a, float b, unsigned &hi, unsigned &lo) {
     hi = pk_bf16(a, b);
-    lo = pk_bf16(a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u));
+    const vsz_b2 h = __builtin_bit_cast(vsz_b2, hi);
+    const vsz_b2 neg0 = __builtin_bit_cast(vsz_b2, vgpr_bits(0x0000bf80u));  // (-1, 0)
+    const vsz_b2 neg1 = __builtin_bit_cast(vsz_b2, vgpr_bits(0xbf800000u));  // (0, -1)
+    lo = pk_bf16(__builtin_amdgcn_fdot2_f32_bf16(h, neg0, a, false), __builtin_amdgcn_fdot2_f32_bf16(h, neg1, b, false));
 }
 
 __device__ __forceinline__ SplitOp split4(float a, float b, float c, float d) {
@@ -1252,6 +1264,30 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
     }
 }
 
+// Adds a tile's LDS region image (rw x rh cells at rx0, ry0) to one plane's device accumulator
+// and clears it, for a 256-thread workgroup: each wave takes every fourth row, its lanes
+// consecutive columns, so no cell index is divided.
+template <bool EXACT>
+__device__ __forceinline__ void flush_region(const FilterArgs &a, long long pb, int *cell_i, float *cell_f, int rx0,
+                                             int ry0, int rw, int rh) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int y = wave; y < rh; y += 4) {
+        const long long row = pb + (long long)(ry0 + y) * a.stride + rx0;
+        for (int x = lane; x < rw; x += 64) {
+            const int i = y * rw + x;
+            if constexpr (EXACT) {
+                const int vv = cell_i[i];
+                if (vv) atomicAdd((unsigned long long *)a.acc + row + x, (unsigned long long)(long long)vv);
+                cell_i[i] = 0;
+            } else {
+                const float vv = cell_f[i];
+                if (vv != 0.0f) atomicAdd((float *)a.acc + row + x, vv);
+                cell_f[i] = 0.0f;
+            }
+        }
+    }
+}
+
 // Tiled path (radius 0): a 256-thread workgroup owns a tbx x tby tile of reference blocks and
 // accumulates the numerators of all their groups in an LDS image of the region the groups can
 // reach (the tile's search windows). With a non-zero extractor the LDS cells are exact 32-bit
@@ -1341,18 +1377,7 @@ __device__ __forceinline__ void filter_tile_body_vec(const FilterArgs &a, int ti
             if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < cells; i += 256) {
-            const long long at = pb + (long long)(ry0 + i / rw) * a.stride + rx0 + i % rw;
-            if constexpr (EXACT) {
-                const int vv = cell_i[i];
-                if (vv) atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)(long long)vv);
-                cell_i[i] = 0;
-            } else {
-                const float vv = cell_f[i];
-                if (vv != 0.0f) atomicAdd((float *)a.acc + at, vv);
-                cell_f[i] = 0.0f;
-            }
-        }
+        flush_region<EXACT>(a, pb, cell_i, cell_f, rx0, ry0, rw, ry1 - ry0);
         __syncthreads();
     }
 }
@@ -1367,11 +1392,16 @@ __device__ __forceinline__ void filter_tile_body_vec(const FilterArgs &a, int ti
 // consecutive stages chain in registers; where the contracted axis moves to the other side, the
 // data goes once through a per-wave LDS buffer. Indices: g group, i rank, j row, c column.
 //   forward: F1 right (c), F2 left (j), LDS, F3 left (i);  inverse: I3 left (i), LDS, I1 right (c),
-//   I2 left (j).
+//   I2 right (j).
+// A lane's D fragment of a product P is its A fragment of P^T (D[4q+e][l%16] = P^T[l%16][4q+e]),
+// so I2 multiplies I1's result from the right as the transpose: its rows become (g, c) and the
+// contraction runs over (i parity, j); each lane ends with four adjacent pixels of one row.
 // Each product is compensated split-BF16 (mfma_split). The lane's share of diag(X, X) is the same
 // 4 values whether it is the A operand of a left
 // product or the B operand of a right product: (q/2 == (l%16)/8) ? X[l%8][4(q%2) + e] : 0, with
-// X = M forward and X = M^T inverse.
+// X = M forward and X = M^T inverse (X = M^T / 4096 in I3, which carries the 3-D round trip's
+// 1/4096 of the shrinkage: a power of two, so the scaling is exact and in the same place as a
+// separate multiply would be).
 constexpr int kPairSJ = 8, kPairSI = 68, kPairSG = 552;  // LDS strides (floats), at most 2-way bank conflicts
 constexpr int kPairBufFloats = 2 * kPairSG;               // per wave; a multiple of 4 (16-byte rows)
 
@@ -1390,19 +1420,20 @@ __device__ __forceinline__ vsz_f4 mfma_split(const SplitOp &A, const SplitOp &B)
 __device__ __forceinline__ SplitOp split_f4(vsz_f4 v) { return split4(v[0], v[1], v[2], v[3]); }
 
 struct PairConsts {
-    SplitOp fwd, inv;
+    SplitOp fwd, inv, inv_s;  // M, M^T, M^T / 4096
 };
 
 __device__ __forceinline__ PairConsts pair_consts(int l) {
     const int q = l >> 4, r = l & 7, e0 = 4 * (q & 1);
     const bool same = (q >> 1) == ((l & 15) >> 3);
-    float f[4], v[4];
+    float f[4], v[4], vs[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         f[e] = same ? kDct8x4[r * 8 + e0 + e] : 0.0f;    // M[r][e0 + e]
         v[e] = same ? kDct8x4[(e0 + e) * 8 + r] : 0.0f;  // M^T[r][e0 + e]
+        vs[e] = v[e] * (1.0f / 4096.0f);
     }
-    return PairConsts{split4(f[0], f[1], f[2], f[3]), split4(v[0], v[1], v[2], v[3])};
+    return PairConsts{split4(f[0], f[1], f[2], f[3]), split4(v[0], v[1], v[2], v[3]), split4(vs[0], vs[1], vs[2], vs[3])};
 }
 
 // Forward 3-D DCT of a group pair. xa[t] holds this lane's row of the F1 operand for rank pair
@@ -1432,14 +1463,15 @@ __device__ __forceinline__ void pair_forward(const SplitOp (&xa)[4], const PairC
     lds_order();
 }
 
-// Inverse of pair_forward. zz[s] in the layout pair_forward returns; out[t][e] = pixel
-// Y[g = (l%16)/8][i = 2t + q/2][j = 4(q%2) + e][c = l%8].
+// Inverse of pair_forward, including the factor 1/4096. zz[s] in the layout pair_forward
+// returns; out[t][e] = pixel Y[g = q/2][i = 2t + (l%16)/8][j = l%8][c = 4(q%2) + e], the
+// positions pair_load read.
 __device__ __forceinline__ void pair_inverse(const vsz_f4 (&zz)[4], const PairConsts &K, float *tb, int l,
                                              vsz_f4 (&out)[4]) {
     const int q = l >> 4, h = (l & 15) >> 3, c = l & 7;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const vsz_f4 v3 = mfma_split(K.inv, split_f4(zz[s]));  // I3: along i
+        const vsz_f4 v3 = mfma_split(K.inv_s, split_f4(zz[s]));  // I3: along i, with 1/4096
         float *w = tb + (q >> 1) * kPairSG + 4 * (q & 1) * kPairSI + (2 * s + h) * kPairSJ + c;
 #pragma unroll
         for (int e = 0; e < 4; ++e) w[e * kPairSI] = v3[e];
@@ -1451,7 +1483,7 @@ __device__ __forceinline__ void pair_inverse(const vsz_f4 (&zz)[4], const PairCo
         const vsz_f4 a = *reinterpret_cast<const vsz_f4 *>(tb + (q >> 1) * kPairSG + (2 * t + h) * kPairSI +
                                                             c * kPairSJ + 4 * (q & 1));
         const vsz_f4 v1 = mfma_split(split_f4(a), K.inv);    // I1: rows along c
-        out[t] = mfma_split(K.inv, split_f4(v1));            // I2: columns along j
+        out[t] = mfma_split(split_f4(v1), K.inv);            // I2: along j, on I1's result transposed
     }
     lds_order();
 }
@@ -1469,29 +1501,27 @@ __device__ __forceinline__ int half_wave_sum(int x) {
     return x;
 }
 
-// The F1 operand rows of a group pair (and of its basic estimate in the final pass), gathered
-// with one 16-byte load per rank pair from the plane's buffer.
+// The F1 operand rows of a group pair (and of its basic estimate in the final pass), loaded
+// with one 16-byte load per rank pair from the plane's buffer: v[t] = X[g = q/2][i = 2t + (l%16)/8]
+// [j = l%8][c = 4(q%2) .. +3] at patch (px[t], py[t], pz[t]).
 template <bool FINAL>
-__device__ __forceinline__ void pair_gather(const FilterArgs &a, int p, const int4 (&pa)[4], int l, SplitOp (&xv)[4],
-                                            SplitOp (&xr)[4]) {
+__device__ __forceinline__ void pair_load(const FilterArgs &a, int p, const int (&px)[4], const int (&py)[4],
+                                          const int (&pz)[4], int l, vsz_f4 (&v)[4], vsz_f4 (&r)[4]) {
     const unsigned bytes = (unsigned)(a.plane_elems * 4);
     const PlaneBuf bref = make_plane_buf(a.clip0 + (long long)p * a.plane_elems, bytes);
     const PlaneBuf bnoisy = FINAL ? make_plane_buf(a.clip1 + (long long)p * a.plane_elems, bytes) : bref;
     const int j = l & 7, c0 = 4 * ((l >> 4) & 1);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int off = (int)(pa[t].z * a.ts + (long long)(pa[t].y + j) * a.stride + pa[t].x + c0) * 4;
-        const vsz_f4 v = __builtin_bit_cast(vsz_f4, __builtin_amdgcn_raw_buffer_load_b128(bnoisy.rsrc, off, 0, 0));
-        xv[t] = split_f4(v);
-        if constexpr (FINAL) {
-            const vsz_f4 r = __builtin_bit_cast(vsz_f4, __builtin_amdgcn_raw_buffer_load_b128(bref.rsrc, off, 0, 0));
-            xr[t] = split_f4(r);
-        }
+        const int off = (int)(pz[t] * a.ts + (long long)(py[t] + j) * a.stride + px[t] + c0) * 4;
+        v[t] = __builtin_bit_cast(vsz_f4, __builtin_amdgcn_raw_buffer_load_b128(bnoisy.rsrc, off, 0, 0));
+        if constexpr (FINAL) r[t] = __builtin_bit_cast(vsz_f4, __builtin_amdgcn_raw_buffer_load_b128(bref.rsrc, off, 0, 0));
     }
 }
 
 // Shrinkage of a group pair's coefficients (src/bm3d.cu:416-546): hard threshold, or Wiener
-// with the basic estimate's coefficients r. Returns the weight 1/k of the lane's group (q/2).
+// with the basic estimate's coefficients r; the 1/4096 of the kept values is applied by
+// pair_inverse. Returns the weight 1/k of the lane's group (q/2).
 template <bool FINAL>
 __device__ __forceinline__ float pair_shrink(vsz_f4 (&z)[4], const vsz_f4 (&r)[4], float sigma, int l) {
     // the 3-D DC (i = 0, j = 0, c = 0) sits in element 0 of N-tile 0 of lanes 0 and 32
@@ -1506,7 +1536,7 @@ __device__ __forceinline__ float pair_shrink(vsz_f4 (&z)[4], const vsz_f4 (&r)[4
                 const float r2 = r[s][e] * r[s][e];
                 float coeff = __fdiv_rn(r2, r2 + s2);
                 if (s == 0 && e == 0 && dc_lane) coeff = 1.0f;
-                z[s][e] = (z[s][e] * coeff) * (1.0f / 4096.0f);
+                z[s][e] = z[s][e] * coeff;
                 k = fmaf_(coeff, coeff, k);
             }
         return __fdiv_rn(1.0f, half_wave_sum(k));
@@ -1519,38 +1549,55 @@ __device__ __forceinline__ float pair_shrink(vsz_f4 (&z)[4], const vsz_f4 (&r)[4
                 const float thr = (s == 0 && e == 0 && dc_lane) ? 0.0f : sigma;
                 const bool keep = fabsf(z[s][e]) >= thr;
                 kept += keep ? 1 : 0;
-                z[s][e] = keep ? z[s][e] * (1.0f / 4096.0f) : 0.0f;
+                z[s][e] = keep ? z[s][e] : 0.0f;
             }
         return __fdiv_rn(1.0f, (float)half_wave_sum(kept));
     }
 }
 
-// One group pair, gather to filtered pixels. pa: patches (g = q/2, i = 2t + (l%16)/8) of the
-// gather; returns out[t][e] = pixel (g = (l%16)/8, i = 2t + q/2, j = 4(q%2) + e, c = l%8) and the
-// weight of group (l%16)/8.
+// One group pair, loaded pixels (pair_load) to filtered pixels: returns out[t][e] = pixel
+// (g = q/2, i = 2t + (l%16)/8, j = l%8, c = 4(q%2) + e), the positions pair_load read, and the
+// weight of group q/2.
 template <bool FINAL>
-__device__ __forceinline__ float pair_filter(const FilterArgs &a, int p, const int4 (&pa)[4], const PairConsts &K,
-                                             float *tb, int l, vsz_f4 (&out)[4]) {
+__device__ __forceinline__ float pair_filter(const vsz_f4 (&v)[4], const vsz_f4 (&vr)[4], float sigma,
+                                             const PairConsts &K, float *tb, int l, vsz_f4 (&out)[4]) {
     SplitOp xv[4], xr[4];
-    pair_gather<FINAL>(a, p, pa, l, xv, xr);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        xv[t] = split_f4(v[t]);
+        if constexpr (FINAL) xr[t] = split_f4(vr[t]);
+    }
     vsz_f4 z[4], r[4];
     pair_forward(xv, K, tb, l, z);
     if constexpr (FINAL) pair_forward(xr, K, tb, l, r);
-    const float w = pair_shrink<FINAL>(z, r, a.sigma[p], l);
+    const float w = pair_shrink<FINAL>(z, r, sigma, l);
     pair_inverse(z, K, tb, l, out);
-    return __shfl(w, ((l & 15) >> 3) * 32);
+    return w;
 }
 
-// Positions of a pair's patches: the gather's (g = q/2, i = 2t + (l%16)/8) and the output's
-// (g = (l%16)/8, i = 2t + q/2).
-__device__ __forceinline__ void pair_positions(const int4 *m0, const int4 *m1, int l, int4 (&pa)[4], int4 (&pb)[4]) {
+// Positions of a pair's patches in the lane's layout: (g = q/2, i = 2t + (l%16)/8).
+__device__ __forceinline__ void pair_positions(const int4 *m0, const int4 *m1, int l, int (&px)[4], int (&py)[4],
+                                               int (&pz)[4]) {
     const int q = l >> 4, h = (l & 15) >> 3;
     const int4 *ma = (q >> 1) ? m1 : m0;
-    const int4 *mb = h ? m1 : m0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        pa[t] = ma[2 * t + h];
-        pb[t] = mb[2 * t + (q >> 1)];
+        const int4 m = ma[2 * t + h];
+        px[t] = m.x;
+        py[t] = m.y;
+        pz[t] = m.z;
+    }
+}
+
+// The same for the tiled path (radius 0, every match in slot 0): x and y only, one 8-byte load each.
+__device__ __forceinline__ void pair_positions_xy(const int4 *m0, const int4 *m1, int l, int (&px)[4], int (&py)[4]) {
+    const int q = l >> 4, h = (l & 15) >> 3;
+    const int4 *ma = (q >> 1) ? m1 : m0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int2 m = *reinterpret_cast<const int2 *>(ma + 2 * t + h);
+        px[t] = m.x;
+        py[t] = m.y;
     }
 }
 
@@ -1558,7 +1605,7 @@ __device__ __forceinline__ void pair_positions(const int4 *m0, const int4 *m1, i
 // waves per workgroup, each filtering two groups at a time.
 template <bool FINAL, bool EXACT>
 __device__ __forceinline__ void filter_tile_body_mfma(const FilterArgs &a, int tile_x, int tile_y, float *smem) {
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, h = (l & 15) >> 3, c = l & 7;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, q = l >> 4, j = l & 7, c0 = 4 * (q & 1);
     float *tb = smem + wave * kPairBufFloats;
     int *cell_i = reinterpret_cast<int *>(smem + 4 * kPairBufFloats);
     float *cell_f = smem + 4 * kPairBufFloats;
@@ -1579,77 +1626,89 @@ __device__ __forceinline__ void filter_tile_body_mfma(const FilterArgs &a, int t
     for (int i = threadIdx.x; i < cells; i += 256) cell_i[i] = 0;
     __syncthreads();
 
+    // positions of the pair at group index `base` of the tile (its second group clamped to the
+    // tile, and a pair past the tile reads the last group: in bounds, never used)
+    const auto positions = [&](int base, int (&px)[4], int (&py)[4]) {
+        const int g0 = min(base, ngroups - 1), g1 = min(base + 1, ngroups - 1);
+        const long long gid0 = (long long)(by0 + g0 / nbx_t) * a.nbx + bx0 + g0 % nbx_t;
+        const long long gid1 = (long long)(by0 + g1 / nbx_t) * a.nbx + bx0 + g1 % nbx_t;
+        pair_positions_xy(a.matches + gid0 * 8, a.matches + gid1 * 8, l, px, py);
+    };
+    const int zero4[4] = {0, 0, 0, 0};  // radius 0: every match is in slot 0
+
     for (int p = 0; p < a.num_planes; ++p) {
         if (!((a.active_mask >> p) & 1)) continue;
         const long long pb = (long long)p * a.acc_plane_elems;
-        for (int base = wave * 2; base < ngroups; base += 8) {
-            const int g0 = base, g1 = min(base + 1, ngroups - 1);
-            const long long gid0 = (long long)(by0 + g0 / nbx_t) * a.nbx + bx0 + g0 % nbx_t;
-            const long long gid1 = (long long)(by0 + g1 / nbx_t) * a.nbx + bx0 + g1 % nbx_t;
-            int4 pa[4], po[4];
-            pair_positions(a.matches + gid0 * 8, a.matches + gid1 * 8, l, pa, po);
+        int base = wave * 2;
+        if (base < ngroups) {  // (no early exit: every wave reaches the barriers below)
+            // software pipeline: the positions of the next pair are in flight while this pair is
+            // filtered (pixel prefetch as well costs more occupancy than it hides)
+            int cx[4], cy[4], nx[4], ny[4];
+            positions(base, cx, cy);
+            for (;;) {
+                const bool more = base + 8 < ngroups;
+                vsz_f4 cv[4], cr[4];
+                pair_load<FINAL>(a, p, cx, cy, zero4, l, cv, cr);
+                if (more) positions(base + 8, nx, ny);
+                vsz_f4 out[4];
+                const float w = pair_filter<FINAL>(cv, cr, a.sigma[p], K, tb, l, out);
+                if (base + (q >> 1) < ngroups) {  // else the pair's second group is past the tile
+                    const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
+                    const float wq = (w + E) - E;
+                    const unsigned long long wfix = EXACT ? (unsigned long long)to_fixed(wq, a.extractor_exp) : 0ull;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) pa[t].z = po[t].z = 0;  // radius 0: every match is in slot 0
-            vsz_f4 out[4];
-            const float w = pair_filter<FINAL>(a, p, pa, K, tb, l, out);
-            if (base + h >= ngroups) continue;  // the pair's second group is past the tile
-            const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
+                    for (int t = 0; t < 4; ++t) {
+                        const int mx = cx[t], my = cy[t];
+                        const bool inside = mx >= rx0 && mx + 8 <= rx1 && my >= ry0 && my + 8 <= ry1;
+                        const int cell0 = (my + j - ry0) * rw + mx + c0 - rx0;
+                        const long long at0 = pb + (long long)(my + j) * a.stride + mx + c0;
+                        if constexpr (EXACT) {
+                            // the common case tests the four terms' largest magnitude once (a
+                            // NaN-propagating maximum, so a NaN term takes the per-term path)
+                            float sq[4];
+                            float big = 0.0f;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int mx = po[t].x, my = po[t].y;
-                const bool inside = mx >= rx0 && mx + 8 <= rx1 && my >= ry0 && my + 8 <= ry1;
-                const int cell0 = (my + 4 * (q & 1) - ry0) * rw + mx + c - rx0;
-                if constexpr (EXACT) {
-                    // the common case tests the four terms' largest magnitude once (a
-                    // NaN-propagating maximum, so a NaN term takes the per-term path)
-                    float sq[4];
-                    float big = 0.0f;
+                            for (int e = 0; e < 4; ++e) {
+                                sq[e] = scaled_term(ws, out[t][e]);
+                                big = __builtin_elementwise_maximum(big, fabsf(sq[e]));
+                            }
+                            if (inside && big < guard_f) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        sq[e] = scaled_term(ws, out[t][e]);
-                        big = __builtin_elementwise_maximum(big, fabsf(sq[e]));
-                    }
-                    if (inside && big < guard_f) {
+                                for (int e = 0; e < 4; ++e) atomicAdd(cell_i + cell0 + e, (int)sq[e]);
+                            } else {
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) atomicAdd(cell_i + cell0 + e * rw, (int)sq[e]);
-                    } else {
+                                for (int e = 0; e < 4; ++e) {
+                                    if (inside && fabsf(sq[e]) < guard_f) atomicAdd(cell_i + cell0 + e, (int)sq[e]);
+                                    else atomicAdd((unsigned long long *)a.acc + at0 + e, (unsigned long long)fixed_from_scaled(sq[e]));
+                                }
+                            }
+                        } else {
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            const long long at = pb + (long long)(my + 4 * (q & 1) + e) * a.stride + mx + c;
-                            if (inside && fabsf(sq[e]) < guard_f) atomicAdd(cell_i + cell0 + e * rw, (int)sq[e]);
-                            else atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)fixed_from_scaled(sq[e]));
+                            for (int e = 0; e < 4; ++e) {
+                                const float qv = (w * out[t][e] + E) - E;
+                                if (inside) atomicAdd(cell_f + cell0 + e, qv);
+                                else atomicAdd((float *)a.acc + at0 + e, qv);
+                            }
+                        }
+                        // weights: one lane per patch adds it at the top-left corner
+                        if (j == 0 && (q & 1) == 0) {
+                            const long long at = pb + a.ts + (long long)my * a.stride + mx;
+                            if constexpr (EXACT) atomicAdd((unsigned long long *)a.acc + at, wfix);
+                            else atomicAdd((float *)a.acc + at, wq);
                         }
                     }
-                } else {
+                }
+                if (!more) break;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const float qv = (w * out[t][e] + E) - E;
-                        if (inside) atomicAdd(cell_f + cell0 + e * rw, qv);
-                        else atomicAdd((float *)a.acc + pb + (long long)(my + 4 * (q & 1) + e) * a.stride + mx + c, qv);
-                    }
+                for (int t = 0; t < 4; ++t) {
+                    cx[t] = nx[t];
+                    cy[t] = ny[t];
                 }
-                // weights: one lane per patch adds it at the top-left corner
-                if (c == 0 && (q & 1) == 0) {
-                    const long long at = pb + a.ts + (long long)my * a.stride + mx;
-                    if constexpr (EXACT)
-                        atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)to_fixed((w + E) - E, a.extractor_exp));
-                    else atomicAdd((float *)a.acc + at, (w + E) - E);
-                }
+                base += 8;
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < cells; i += 256) {
-            const long long at = pb + (long long)(ry0 + i / rw) * a.stride + rx0 + i % rw;
-            if constexpr (EXACT) {
-                const int vv = cell_i[i];
-                if (vv) atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)(long long)vv);
-                cell_i[i] = 0;
-            } else {
-                const float vv = cell_f[i];
-                if (vv != 0.0f) atomicAdd((float *)a.acc + at, vv);
-                cell_f[i] = 0.0f;
-            }
-        }
+        flush_region<EXACT>(a, pb, cell_i, cell_f, rx0, ry0, rw, ry1 - ry0);
         __syncthreads();
     }
 }
@@ -1665,35 +1724,37 @@ __device__ __forceinline__ void filter_tile_body(const FilterArgs &a, int tile_x
 template <bool FINAL, bool EXACT>
 __global__ __launch_bounds__(64) void k_filter_mfma(FilterArgs a) {
     __shared__ __align__(16) float tbuf[kPairBufFloats];
-    const int l = threadIdx.x, q = l >> 4, h = (l & 15) >> 3, c = l & 7;
+    const int l = threadIdx.x, q = l >> 4, j = l & 7, c0 = 4 * (q & 1);
     const long long gid0 = min((long long)blockIdx.x * 2, a.nblocks - 1);
     const long long gid1 = min((long long)blockIdx.x * 2 + 1, a.nblocks - 1);
-    const bool valid = (long long)blockIdx.x * 2 + h < a.nblocks;
-    int4 pa[4], po[4];
-    pair_positions(a.matches + gid0 * 8, a.matches + gid1 * 8, l, pa, po);
+    const bool valid = (long long)blockIdx.x * 2 + (q >> 1) < a.nblocks;
+    int px[4], py[4], pz[4];
+    pair_positions(a.matches + gid0 * 8, a.matches + gid1 * 8, l, px, py, pz);
     const PairConsts K = pair_consts(l);
     const float E = a.extractor;
     for (int p = 0; p < a.num_planes; ++p) {
         if (!((a.active_mask >> p) & 1)) continue;
-        vsz_f4 out[4];
-        const float w = pair_filter<FINAL>(a, p, pa, K, tbuf, l, out);
+        vsz_f4 v[4], vr[4], out[4];
+        pair_load<FINAL>(a, p, px, py, pz, l, v, vr);
+        const float w = pair_filter<FINAL>(v, vr, a.sigma[p], K, tbuf, l, out);
         if (!valid) continue;
         const long long pb = (long long)p * a.acc_plane_elems;
         const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
+        const float wq = (w + E) - E;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const long long pbase = pb + (long long)po[t].z * 2 * a.ts + (long long)po[t].y * a.stride + po[t].x;
+            const long long pbase = pb + (long long)pz[t] * 2 * a.ts + (long long)py[t] * a.stride + px[t];
+            const long long at0 = pbase + (long long)j * a.stride + c0;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const long long at = pbase + (long long)(4 * (q & 1) + e) * a.stride + c;
                 if constexpr (EXACT)
-                    atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)fixed_from_scaled(scaled_term(ws, out[t][e])));
-                else atomicAdd((float *)a.acc + at, (w * out[t][e] + E) - E);
+                    atomicAdd((unsigned long long *)a.acc + at0 + e, (unsigned long long)fixed_from_scaled(scaled_term(ws, out[t][e])));
+                else atomicAdd((float *)a.acc + at0 + e, (w * out[t][e] + E) - E);
             }
-            if (c == 0 && (q & 1) == 0) {
+            if (j == 0 && (q & 1) == 0) {
                 if constexpr (EXACT)
-                    atomicAdd((unsigned long long *)a.acc + pbase + a.ts, (unsigned long long)to_fixed((w + E) - E, a.extractor_exp));
-                else atomicAdd((float *)a.acc + pbase + a.ts, (w + E) - E);
+                    atomicAdd((unsigned long long *)a.acc + pbase + a.ts, (unsigned long long)to_fixed(wq, a.extractor_exp));
+                else atomicAdd((float *)a.acc + pbase + a.ts, wq);
             }
         }
     }
