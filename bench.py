@@ -204,8 +204,9 @@ def profile_target(args):
 
 def bench_batch(args):
     """64 independent Gray planes of seeded uneven sizes and sigmas, two-pass BM3D (bs 4, R 12,
-    extractor 3) as one batched plan: one launch per stage, replayed as one hipGraph, filter
-    tiles from the autotuner. Compared with the same planes run one after another."""
+    extractor 3) as one batched plan replayed as one hipGraph: filter tiles, schedule (one
+    persistent launch, or one launch per stage) and matcher from the autotuner. Both schedules
+    are also timed on their own, and the planes run one after another."""
     import torch
     from vszipcu import device, synth
     rng = np.random.default_rng(4)
@@ -261,6 +262,21 @@ def bench_batch(args):
     torch.cuda.synchronize()
     dt_seq = (time.perf_counter() - t0) / max(1, args.steps // 2)
     tiles = sorted({b.tile(i, 0) for i in range(64)} | {b.tile(i, 1) for i in range(64)})
+    chosen, shared_cols = b.schedule()
+    sched_ms = {}
+    for sched in ("staged", "persistent"):
+        bs_ = device.Batch(planes, block_step=4, bm_range=12, extractor_exp=3, autotune=True, schedule=sched)
+        ws_ = torch.zeros(bs_.workspace_size, dtype=torch.uint8, device="cuda")
+        os_ = [torch.zeros_like(t) for t in noisy]
+        for _ in range(1 + args.warmup):
+            bs_.run(noisy, os_, ws_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bs_.run(noisy, os_, ws_)
+        torch.cuda.synchronize()
+        sched_ms[sched] = round((time.perf_counter() - t0) / args.steps * 1e3, 3)
+        del bs_, ws_, os_
     ssd = sum(192.0 * 2 * ssd_candidates(w, h, 4, 12) for w, h in sizes)
     tf = sum(128.0 * num_blocks(w, h, 4) * (384 + 576) for w, h in sizes)
     return {"metric": "BM3D batched uneven planes Mpix/s", "value": round(px / dt / 1e6, 3),
@@ -269,7 +285,10 @@ def bench_batch(args):
                       "data": "synthetic",
                       "config": {"workload": "64 Gray planes, sizes {320..3840}x{240..2160} (seed 4), sigma U[1,20], "
                                              "two-pass, bs 4, R 12, extractor 3, one hipGraph per run",
-                                 "mpix": round(px / 1e6, 2), "sequential_ms": round(dt_seq * 1e3, 3),
+                                 "mpix": round(px / 1e6, 2), "schedule": chosen,
+                                 "matcher": "shared-column" if shared_cols else "rows",
+                                 "staged_ms": sched_ms["staged"], "persistent_ms": sched_ms["persistent"],
+                                 "sequential_ms": round(dt_seq * 1e3, 3),
                                  "sequential_mpix_s": round(px / dt_seq / 1e6, 3),
                                  "setup_s": round(t_setup, 2), "autotuned_tiles": tiles},
             "step_roofline": step_roofline(ssd, tf, dt)}

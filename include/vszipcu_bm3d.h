@@ -125,9 +125,19 @@ typedef struct vsz_bm3d_batch_params {
     int metric, t2d, t1d;        /* t2d = t1d = 0 (dct) */
     int transform_precision;     /* as in vsz_bm3d_kparams */
     int two_pass;                /* 1: basic, then Wiener with ref = basic; 0: basic only */
-    int autotune;                /* 1: time every fitting filter tile shape per plane shape on the
-                                    first run and keep the fastest; 0: cost model */
+    int autotune;                /* 1: on the first run, time every fitting filter tile shape per
+                                    plane shape, and whole runs of every allowed (schedule, matcher)
+                                    pair, and keep the fastest; 0: cost model and preferences */
+    int schedule;                /* VSZ_BATCH_STAGED: one launch per stage (zeroing, matching,
+                                    filtering, finishing of all planes, per pass) in one graph;
+                                    VSZ_BATCH_PERSISTENT: the whole batch in one persistent launch
+                                    (extractor_exp > 0, ssd or sad); VSZ_BATCH_AUTO: the autotuner
+                                    times both (staged without autotune) */
 } vsz_bm3d_batch_params;
+
+#define VSZ_BATCH_STAGED 0
+#define VSZ_BATCH_PERSISTENT 1
+#define VSZ_BATCH_AUTO 2
 
 typedef struct vsz_bm3d_batch vsz_bm3d_batch;
 
@@ -142,6 +152,13 @@ int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *co
                        void *stream);
 /* Filter tile (in reference blocks) chosen for plane i, pass 0 (basic) or 1 (final). */
 int vsz_bm3d_batch_tile(const vsz_bm3d_batch *b, int plane, int pass, int *tbx, int *tby);
+/* Schedule (VSZ_BATCH_STAGED or VSZ_BATCH_PERSISTENT) and matcher (1: shared-column block_step-4
+ * matcher, 0: row matcher) of the batch; final after its first run. */
+int vsz_bm3d_batch_schedule(const vsz_bm3d_batch *b, int *schedule, int *shared_cols);
+/* Waits for the work queued on `stream`, then reports whether any dependency wait of the batch's
+ * persistent launches (since its creation) ran into its time bound, i.e. ran before its inputs
+ * were complete: VSZ_OK, or an error. Always VSZ_OK for the staged schedule. */
+int vsz_bm3d_batch_check(vsz_bm3d_batch *b, void *stream);
 void vsz_bm3d_batch_free(vsz_bm3d_batch *b);
 
 /* =============================================================================================

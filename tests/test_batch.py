@@ -1,5 +1,6 @@
-"""Batched uneven planes (vsz_bm3d_batch_*): one launch per stage over all planes, replayed as
-a hipGraph, tile shapes from the autotuner or the cost model.
+"""Batched uneven planes (vsz_bm3d_batch_*): one launch per stage over all planes, or the whole
+batch in one persistent launch, replayed as a hipGraph; tile shapes, schedule and matcher from the
+autotuner or the cost model.
 
 With a non-zero extractor the accumulator sums are exact integers, so the batch must give the
 same bits as running each plane alone through vsz_bm3d_run (basic, then Wiener with ref =
@@ -15,6 +16,8 @@ from vszipcu import device, synth
 PLANES = [(64, 48, 64, 5.0), (40, 32, 48, 12.0), (96, 40, 96, 3.5), (37, 29, 40, 18.0), (128, 72, 128, 9.0),
           (200, 120, 208, 1.5)]
 KW = dict(block_step=4, bm_range=12, extractor_exp=3)
+# every (width - 8) a multiple of 4: the shared-column matcher is allowed too
+PLANES_SC = [(64, 48, 64, 5.0), (40, 32, 48, 12.0), (96, 40, 96, 3.5), (128, 72, 128, 9.0), (200, 120, 208, 1.5)]
 
 
 def test_batch_rejects_bad_requests():
@@ -26,6 +29,10 @@ def test_batch_rejects_bad_requests():
         device.Batch([(64, 48, 60, 5.0)], **KW)          # stride < width
     with pytest.raises(RuntimeError, match="too large for the LDS-staged matcher"):
         device.Batch([(640, 480, 640, 5.0)], block_step=8, bm_range=120)
+    with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0 and the ssd or sad"):
+        device.Batch([(64, 48, 64, 5.0)], schedule="persistent", metric="zssd", **KW)
+    with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0 and the ssd or sad"):
+        device.Batch([(64, 48, 64, 5.0)], schedule="persistent", block_step=4, bm_range=12, extractor_exp=0)
 
 
 def _inputs(torch, planes, seed):
@@ -51,28 +58,44 @@ def _single(torch, plane, noisy, precision="fp32"):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("autotune,precision", [(False, "fp32"), (True, "fp32"), (True, "split-bf16")])
-def test_batch_equals_single_plane_runs(torch_gpu, autotune, precision):
+@pytest.mark.parametrize("planes", ["uneven", "shared_cols"])
+@pytest.mark.parametrize("autotune,precision,schedule", [(False, "fp32", "staged"), (True, "fp32", "staged"),
+                                                         (True, "split-bf16", "staged"), (False, "fp32", "persistent"),
+                                                         (True, "split-bf16", "persistent"), (True, "fp32", "auto")])
+def test_batch_equals_single_plane_runs(torch_gpu, planes, autotune, precision, schedule):
     torch = torch_gpu
-    print(f"batch: {len(PLANES)} planes autotune={autotune} precision={precision}", flush=True)
-    b = device.Batch(PLANES, autotune=autotune, precision=precision, **KW)
-    noisy = _inputs(torch, PLANES, 40)
-    outs = [torch.full((h, st), -1.0, dtype=torch.float32, device="cuda") for (w, h, st, _) in PLANES]
+    P = PLANES if planes == "uneven" else PLANES_SC
+    print(f"batch: {len(P)} {planes} planes autotune={autotune} precision={precision} schedule={schedule}", flush=True)
+    b = device.Batch(P, autotune=autotune, precision=precision, schedule=schedule, **KW)
+    noisy = _inputs(torch, P, 40)
+    outs = [torch.full((h, st), -1.0, dtype=torch.float32, device="cuda") for (w, h, st, _) in P]
     work = torch.zeros(b.workspace_size, dtype=torch.uint8, device="cuda")
     b.run(noisy, outs, work)
     torch.cuda.synchronize()
-    for i, p in enumerate(PLANES):
+    b.check()                         # no dependency wait of a persistent launch ran out
+    for i, p in enumerate(P):
         want = _single(torch, p, noisy[i], precision)
         got = outs[i].cpu().numpy()[:, :p[0]]
         assert np.array_equal(got, want), f"plane {i} {p}"
         tx, ty = b.tile(i, 0)
         assert tx * ty >= 1
+    sched, sc = b.schedule()
+    if schedule != "auto":
+        assert sched == schedule
+    if planes == "uneven":
+        assert not sc                 # a width with (W - 8) % 4 != 0 rules the shared-column matcher out
+    elif not autotune:
+        assert sc                     # preferred where allowed
 
 
 @pytest.mark.gpu
-def test_batch_graph_replays_follow_inputs(torch_gpu):
+@pytest.mark.parametrize("schedule", ["staged", "persistent"])
+def test_batch_graph_replays_follow_inputs(torch_gpu, schedule):
+    """Replays with inputs changed in place; for the persistent launch this also checks that every
+    launch leaves its work queue empty for the next."""
     torch = torch_gpu
-    b = device.Batch(PLANES, autotune=True, **KW)
+    print(f"batch replays: schedule={schedule}", flush=True)
+    b = device.Batch(PLANES, autotune=True, schedule=schedule, **KW)
     noisy = _inputs(torch, PLANES, 60)
     outs = [torch.zeros((h, st), dtype=torch.float32, device="cuda") for (w, h, st, _) in PLANES]
     work = torch.zeros(b.workspace_size, dtype=torch.uint8, device="cuda")
@@ -90,5 +113,6 @@ def test_batch_graph_replays_follow_inputs(torch_gpu):
     outs2 = [torch.zeros_like(o) for o in outs]
     b.run(noisy, outs2, work)
     torch.cuda.synchronize()
+    b.check()
     for o, o2 in zip(outs, outs2):
         assert torch.equal(o, o2)

@@ -252,11 +252,11 @@ def _oracle_two_pass(orc, i, nz):
     return _ORACLE_BATCH[i]
 
 
-@pytest.mark.parametrize("autotune", [True, False])
-def test_batch_planes_against_oracle(orc, torch_gpu, autotune):
+@pytest.mark.parametrize("autotune,schedule", [(True, "auto"), (False, "staged"), (False, "persistent")])
+def test_batch_planes_against_oracle(orc, torch_gpu, autotune, schedule):
     torch = torch_gpu
-    print(f"batch vs oracle: {len(BATCH_PLANES)} planes autotune={autotune}", flush=True)
-    b = device.Batch(BATCH_PLANES, block_step=4, bm_range=12, extractor_exp=3, autotune=autotune)
+    print(f"batch vs oracle: {len(BATCH_PLANES)} planes autotune={autotune} schedule={schedule}", flush=True)
+    b = device.Batch(BATCH_PLANES, block_step=4, bm_range=12, extractor_exp=3, autotune=autotune, schedule=schedule)
     noisy, outs = [], []
     for i, (w, h, st, sg) in enumerate(BATCH_PLANES):
         _, nz = synth.gray_frame(w, h, sg, seed=200 + i)
@@ -265,6 +265,7 @@ def test_batch_planes_against_oracle(orc, torch_gpu, autotune):
     work = torch.zeros(b.workspace_size, dtype=torch.uint8, device="cuda")
     b.run(noisy, outs, work)
     torch.cuda.synchronize()
+    b.check()
     for i, (w, h, st, sg) in enumerate(BATCH_PLANES):
         want = _oracle_two_pass(orc, i, noisy[i].cpu().numpy())
         got = outs[i].cpu().numpy()

@@ -1918,6 +1918,135 @@ __global__ __launch_bounds__(256) void k_finish_batch(const BatchTable *__restri
     finish_body<EXACT>(a, local % tx, (local / tx) % ty, local / (tx * ty), sD, sH);
 }
 
+// ---- the whole batch in one persistent launch --------------------------------------------------
+// Work items (accumulator zeroing chunks, matcher tiles, filter tiles and finishing tiles of every
+// plane and pass) are handed out in one fixed order from a device counter. Before it runs, an item
+// waits until the items whose results it reads are complete, counted per (pass, stage, plane):
+//   filter <- this pass's match and zeroing of the plane;  finish <- this pass's filtering;
+//   pass 1 match and zeroing <- pass 0 finishing of the plane (its basic estimate, its
+//   accumulator). Those items always come earlier in the order, so the earliest unfinished item can
+// run and the launch drains with any number of resident workgroups; a plane's stages overlap with
+// other planes' earlier stages. Completion is published with agent-scope release (each thread,
+// then the counter update) and observed with agent-scope acquire, which covers the per-XCD L2s.
+enum : int { kStageZero = 0, kStageMatch = 1, kStageFilter = 2, kStageFinish = 3 };
+constexpr int kZeroChunk = 256 * 16;  // 64-bit words per zeroing item
+constexpr int kMaxSegs = 2 * 4 * kMaxBatch;
+constexpr int kSpinLimit = 1 << 24;  // polls of 256 cycles: a bound far above any legitimate wait
+
+struct BatchSeg {
+    int begin;  // first item of the segment (items of one pass, stage and plane)
+    int pass, stage, plane;
+};
+
+struct PersistTable {
+    int nseg, total;
+    BatchSeg seg[kMaxSegs];
+    int count[2][4][kMaxBatch];  // items per (pass, stage, plane)
+    unsigned long long *acc[2][kMaxBatch];  // per pass and plane
+    long long acc_words[kMaxBatch];
+};
+
+struct BatchQueue {
+    unsigned next, exited;
+    unsigned timeouts;  // waits that reached kSpinLimit (never reset by the kernel)
+    unsigned done[2][4][kMaxBatch];
+};
+
+__device__ __forceinline__ void await_count(const unsigned *cnt, int need, unsigned *timeouts) {
+    for (int spin = 0; spin < kSpinLimit; ++spin) {
+        if (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)need) return;
+        __builtin_amdgcn_s_sleep(4);
+    }
+    atomicAdd(timeouts, 1u);
+}
+
+// Loop shape: one `threadIdx.x == 0` block per iteration (publish the previous item, claim the
+// next), and the item index read back as a wave-uniform scalar, so that the loop exit and the
+// stage dispatch are uniform branches and every barrier is reached by all lanes of every wave.
+// (With the publication at the end of an iteration and the claim at its start, the compiler
+// joined the two lane-0 blocks across the back edge into an outer loop of their own, and the
+// other lanes re-entered the item loop without a new claim.)
+template <int M, bool SC, int T>
+__global__ __launch_bounds__(256) void k_batch_persistent(const BatchTable *__restrict__ tabs,
+                                                          const PersistTable *__restrict__ pt, BatchQueue *q) {
+    // one dynamic LDS block: 16 bytes of loop control, then the stages' 16-byte aligned memory
+    extern __shared__ __align__(16) float lds[];
+    int &s_item = reinterpret_cast<int *>(lds)[0];
+    int &s_last = reinterpret_cast<int *>(lds)[1];
+    float *smem = lds + 4;
+    int prev_pass = -1, prev_stage = 0, prev_plane = 0;  // the item this workgroup finished last
+    for (;;) {
+        if (threadIdx.x == 0) {
+            if (prev_pass >= 0)  // every thread released its writes before the barrier that ended it
+                __hip_atomic_fetch_add(&q->done[prev_pass][prev_stage][prev_plane], 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            s_item = (int)atomicAdd(&q->next, 1u);
+        }
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= pt->total) break;
+        int lo = 0, hi = pt->nseg - 1;  // last segment with begin <= it
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pt->seg[mid].begin <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const int local = it - __builtin_amdgcn_readfirstlane(pt->seg[lo].begin);
+        const int pass = __builtin_amdgcn_readfirstlane(pt->seg[lo].pass);
+        const int stage = __builtin_amdgcn_readfirstlane(pt->seg[lo].stage);
+        const int p = __builtin_amdgcn_readfirstlane(pt->seg[lo].plane);
+        if (threadIdx.x == 0) {
+            if (stage == kStageFilter) {
+                await_count(&q->done[pass][kStageMatch][p], pt->count[pass][kStageMatch][p], &q->timeouts);
+                await_count(&q->done[pass][kStageZero][p], pt->count[pass][kStageZero][p], &q->timeouts);
+            } else if (stage == kStageFinish) {
+                await_count(&q->done[pass][kStageFilter][p], pt->count[pass][kStageFilter][p], &q->timeouts);
+            } else if (pass == 1) {
+                await_count(&q->done[0][kStageFinish][p], pt->count[0][kStageFinish][p], &q->timeouts);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const BatchTable &t = tabs[pass];
+        if (stage == kStageZero) {
+            unsigned long long *z = pt->acc[pass][p];
+            const long long b0 = (long long)local * kZeroChunk;
+            const long long e0 = min(b0 + kZeroChunk, pt->acc_words[p]);
+            for (long long i = b0 + threadIdx.x; i < e0; i += 256) z[i] = 0ull;
+        } else if (stage == kStageMatch) {
+            if constexpr (SC) match_lds4_body<M>(t.m[p], local, smem);
+            else match_lds_body<M>(t.m[p], local, smem);
+        } else if (stage == kStageFilter) {
+            const int tx = t.filter_tiles_x[p];
+            if (pass == 0) filter_tile_body<false, T, T, true>(t.f[p], local % tx, local / tx, smem);
+            else filter_tile_body<true, T, T, true>(t.f[p], local % tx, local / tx, smem);
+        } else {
+            long long (*sD)[72] = reinterpret_cast<long long (*)[72]>(smem);
+            long long (*sH)[64] = reinterpret_cast<long long (*)[64]>(smem + 23 * 72 * 2);
+            const int tx = t.finish_tiles_x[p], ty = t.finish_tiles_y[p];
+            finish_body<true>(t.fin[p], local % tx, (local / tx) % ty, local / (tx * ty), sD, sH);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        prev_pass = pass;
+        prev_stage = stage;
+        prev_plane = p;
+    }
+    // (the last item was published at the top of the iteration that left the loop) the last
+    // workgroup to leave resets the queue for the next launch
+    if (threadIdx.x == 0) s_last = atomicAdd(&q->exited, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (s_last) {
+        unsigned *d = &q->done[0][0][0];
+        for (int i = threadIdx.x; i < 2 * 4 * kMaxBatch; i += 256) d[i] = 0u;
+        if (threadIdx.x == 0) {
+            q->next = 0u;
+            q->exited = 0u;
+        }
+    }
+}
+
 }  // namespace vszk
 
 // ================================================================================================
@@ -2465,7 +2594,9 @@ struct BatchPlane {
     vsz_bm3d_kparams k[2]{};  // basic, final
     Layout L{};
     TilePlan tile[2];
-    size_t acc_off = 0, match_off = 0, basic_off = 0;
+    // per pass: accumulators and match lists of each pass have addresses of their own, so that no
+    // memory is rewritten within a persistent launch after another workgroup may have cached it
+    size_t acc_off[2] = {0, 0}, match_off[2] = {0, 0}, basic_off = 0;
 };
 
 float bits_float(uint32_t u) {
@@ -2498,16 +2629,57 @@ void launch_match_batch(int metric, bool shared_cols, const BatchTable *d, int w
     }
 }
 
+// Sizes the grid (as many workgroups as can be resident at once) when *grid is 0, then launches
+// on s unless size_only (s may be the null stream).
+template <int M, bool SC, int T>
+int launch_persistent_t(const BatchTable *tabs, const PersistTable *pt, BatchQueue *q, size_t lds, int *grid,
+                        bool size_only, hipStream_t s) {
+    auto k = k_batch_persistent<M, SC, T>;
+    if (*grid == 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds) != hipSuccess || per_cu < 1 ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: no occupancy for the persistent kernel");
+        *grid = per_cu * cus;
+    }
+    if (!size_only) hipLaunchKernelGGL(k, dim3(*grid), dim3(256), lds, s, tabs, pt, q);
+    return VSZ_OK;
+}
+
+int launch_persistent(int metric, bool sc, bool mfma, const BatchTable *tabs, const PersistTable *pt, BatchQueue *q,
+                      size_t lds, int *grid, bool size_only, hipStream_t s) {
+    if (metric == 0) {
+        if (sc) return mfma ? launch_persistent_t<0, true, kDctMfma>(tabs, pt, q, lds, grid, size_only, s)
+                            : launch_persistent_t<0, true, 0>(tabs, pt, q, lds, grid, size_only, s);
+        return mfma ? launch_persistent_t<0, false, kDctMfma>(tabs, pt, q, lds, grid, size_only, s)
+                    : launch_persistent_t<0, false, 0>(tabs, pt, q, lds, grid, size_only, s);
+    }
+    if (sc) return mfma ? launch_persistent_t<1, true, kDctMfma>(tabs, pt, q, lds, grid, size_only, s)
+                        : launch_persistent_t<1, true, 0>(tabs, pt, q, lds, grid, size_only, s);
+    return mfma ? launch_persistent_t<1, false, kDctMfma>(tabs, pt, q, lds, grid, size_only, s)
+                : launch_persistent_t<1, false, 0>(tabs, pt, q, lds, grid, size_only, s);
+}
+
 }  // namespace
 
 struct vsz_bm3d_batch {
     vsz_bm3d_batch_params bp{};
     std::vector<BatchPlane> planes;
     bool exact = false;
-    size_t acc_total = 0, work_bytes = 0;
+    size_t acc_total = 0, acc_span = 0, work_bytes = 0;  // acc_span: the accumulators of one pass
     size_t match_lds = 0;
     int match_stride = 0;
     bool shared_cols = false;  // every plane takes the shared-column block_step-4 matcher
+    bool sc_allowed = false;   // ... may take it (the autotuner then times both matchers)
+    size_t lds_rows = 0, lds_sc = 0;
+    int stride_rows = 0, stride_sc = 0;
+    int schedule = VSZ_BATCH_STAGED;  // chosen schedule
+    bool persist_ok = false;          // the persistent form supports these parameters
+    PersistTable *h_pt = nullptr, *d_pt = nullptr;
+    BatchQueue *d_q = nullptr;
+    int persist_grid = 0;
     bool tuned = false;
     BatchTable *h_tab[2] = {nullptr, nullptr};
     BatchTable *d_tab = nullptr;  // [2]
@@ -2527,6 +2699,9 @@ struct vsz_bm3d_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
         if (d_tab) (void)hipFree(d_tab);
+        if (d_pt) (void)hipFree(d_pt);
+        if (d_q) (void)hipFree(d_q);
+        delete h_pt;
         delete h_tab[0];
         delete h_tab[1];
     }
@@ -2567,8 +2742,8 @@ struct vsz_bm3d_batch {
                 const bool last = pass == passes() - 1;
                 const float *clip0 = pass == 0 ? noisy[i] : reinterpret_cast<const float *>(work + pl.basic_off);
                 float *out = last ? dst[i] : reinterpret_cast<float *>(work + pl.basic_off);
-                int4 *matches = reinterpret_cast<int4 *>(work + pl.match_off);
-                void *acc = work + pl.acc_off;
+                int4 *matches = reinterpret_cast<int4 *>(work + pl.match_off[pass]);
+                void *acc = work + pl.acc_off[pass];
                 t.match_off[i] = mo;
                 t.filter_off[i] = fo;
                 t.finish_off[i] = so;
@@ -2641,11 +2816,11 @@ struct vsz_bm3d_batch {
 
     // All launches of one run, on stream s (captured into the graph).
     void enqueue(char *work, hipStream_t s) {
-        const long long n64 = (long long)(acc_total / 8);
+        const long long n64 = (long long)(acc_span / 8);
         const unsigned zgrid = (unsigned)std::min<long long>((n64 + 255) / 256, 8192);
         for (int pass = 0; pass < passes(); ++pass) {
             const BatchTable *d = d_tab + pass;
-            hipLaunchKernelGGL(k_zero, dim3(zgrid), dim3(256), 0, s, (unsigned long long *)work, n64);
+            hipLaunchKernelGGL(k_zero, dim3(zgrid), dim3(256), 0, s, (unsigned long long *)(work + pass * acc_span), n64);
             launch_match_batch(bp.metric, shared_cols, d, match_wgs(), match_lds, s);
             const int fw = filter_wgs(pass);
             const size_t fl = filter_lds(pass);
@@ -2662,7 +2837,87 @@ struct vsz_bm3d_batch {
         }
     }
 
+    void use_matcher(bool sc) {
+        shared_cols = sc;
+        match_lds = sc ? lds_sc : lds_rows;
+        match_stride = sc ? stride_sc : stride_rows;
+        persist_grid = 0;  // the persistent kernel's LDS, so its occupancy, follows the matcher
+    }
+
+    // the persistent kernel's dynamic LDS: 16 bytes of loop control and the largest stage's need
+    size_t persist_lds() const {
+        size_t m = std::max(match_lds, (size_t)(23 * 72 + 23 * 64) * 8);  // finishing: sD, sH (int64)
+        for (int pass = 0; pass < passes(); ++pass) m = std::max(m, filter_lds(pass));
+        return 16 + m;
+    }
+
+    // Items of the persistent launch: per pass, the zeroing, matching, filtering and finishing
+    // segments of every plane, each stage's planes largest first (from the tables' tile counts).
+    void fill_persist(char *work) {
+        PersistTable &t = *h_pt;
+        std::vector<int> order(planes.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return (long long)planes[x].k[0].width * planes[x].k[0].height >
+                   (long long)planes[y].k[0].width * planes[y].k[0].height;
+        });
+        memset(&t.count[0][0][0], 0, sizeof(t.count));
+        t.nseg = 0;
+        int item = 0;
+        for (int pass = 0; pass < passes(); ++pass) {
+            const BatchTable &bt = *h_tab[pass];
+            for (int stage = 0; stage < 4; ++stage)
+                for (int i : order) {
+                    int n = 0;
+                    if (stage == kStageZero) n = (int)((planes[i].L.acc_bytes / 8 + kZeroChunk - 1) / kZeroChunk);
+                    else if (stage == kStageMatch) n = bt.match_off[i + 1] - bt.match_off[i];
+                    else if (stage == kStageFilter) n = bt.filter_off[i + 1] - bt.filter_off[i];
+                    else n = bt.finish_off[i + 1] - bt.finish_off[i];
+                    t.count[pass][stage][i] = n;
+                    if (n == 0) continue;
+                    t.seg[t.nseg++] = BatchSeg{item, pass, stage, i};
+                    item += n;
+                }
+        }
+        t.total = item;
+        for (size_t i = 0; i < planes.size(); ++i) {
+            for (int pass = 0; pass < 2; ++pass)
+                t.acc[pass][i] = reinterpret_cast<unsigned long long *>(work + planes[i].acc_off[pass]);
+            t.acc_words[i] = (long long)(planes[i].L.acc_bytes / 8);
+        }
+    }
+
+    // size_only: only sizes the grid (before a capture)
+    int enqueue_persistent(hipStream_t s, bool size_only = false) {
+        return launch_persistent(bp.metric, shared_cols, bp.transform_precision == 1, d_tab, d_pt, d_q, persist_lds(),
+                                 &persist_grid, size_only, s);
+    }
+
+    // Uploads the tables of the current matcher choice for these pointers.
+    int upload(const float *const *noisy, float *const *dst, char *work, hipStream_t s) {
+        fill_tables(noisy, dst, work);
+        for (int pass = 0; pass < passes(); ++pass)
+            if (hipMemcpyAsync(d_tab + pass, h_tab[pass], sizeof(BatchTable), hipMemcpyHostToDevice, s) != hipSuccess)
+                return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
+        if (persist_ok) {
+            fill_persist(work);
+            if (hipMemcpyAsync(d_pt, h_pt, sizeof(PersistTable), hipMemcpyHostToDevice, s) != hipSuccess)
+                return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
+        }
+        // the pageable sources may be reused as soon as the copies are done
+        if (hipStreamSynchronize(s) != hipSuccess) return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
+        return VSZ_OK;
+    }
+
+    // One whole run of the current schedule and matcher, uncaptured, on stream s.
+    int run_direct(char *work, hipStream_t s) {
+        if (schedule == VSZ_BATCH_PERSISTENT) return enqueue_persistent(s);
+        enqueue(work, s);
+        return VSZ_OK;
+    }
+
     int tune(const float *const *noisy, char *work, hipStream_t s);
+    int tune_schedule(const float *const *noisy, float *const *dst, char *work, hipStream_t s);
 };
 
 // Times every candidate tile shape of the filter kernel on each distinct plane shape (both
@@ -2688,14 +2943,14 @@ int vsz_bm3d_batch::tune(const float *const *noisy, char *work, hipStream_t s) {
                 }
             }
             // matches for this plane (the final pass is timed on the noisy plane as its ref)
-            int4 *matches = reinterpret_cast<int4 *>(work + pl.match_off);
+            int4 *matches = reinterpret_cast<int4 *>(work + pl.match_off[pass]);
             rc = vsz_bm3d_match(&k, noisy[i], reinterpret_cast<int32_t *>(matches), s);
             if (rc) break;
             FilterArgs f{};
             f.matches = matches;
             f.clip0 = noisy[i];
             f.clip1 = noisy[i];
-            f.acc = work + pl.acc_off;
+            f.acc = work + pl.acc_off[pass];
             f.nblocks = pl.L.nblocks;
             f.num_planes = 1;
             f.stride = k.stride;
@@ -2752,6 +3007,65 @@ int vsz_bm3d_batch::tune(const float *const *noisy, char *work, hipStream_t s) {
     return rc;
 }
 
+// Chooses the schedule (one persistent launch or one launch per stage) and, where every plane
+// allows the shared-column matcher, the matcher: with autotune, by timing whole runs of every
+// allowed pair on this data (one warm-up, then the best of two); otherwise by preference.
+int vsz_bm3d_batch::tune_schedule(const float *const *noisy, float *const *dst, char *work, hipStream_t s) {
+    struct Cand {
+        int sched;
+        bool sc;
+    };
+    std::vector<Cand> cands;
+    for (int sc = 1; sc >= 0; --sc) {
+        if (sc ? !sc_allowed : lds_rows == 0) continue;
+        if (bp.schedule != VSZ_BATCH_STAGED && persist_ok) cands.push_back({VSZ_BATCH_PERSISTENT, sc != 0});
+        if (bp.schedule != VSZ_BATCH_PERSISTENT) cands.push_back({VSZ_BATCH_STAGED, sc != 0});
+    }
+    if (cands.empty()) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_run: no schedule fits these parameters");
+    if (!bp.autotune || cands.size() == 1) {
+        // preference: the shared-column matcher, the persistent launch only where asked for
+        Cand pick = cands[0];
+        for (const Cand &c : cands)
+            if (c.sc == cands[0].sc && c.sched == (bp.schedule == VSZ_BATCH_PERSISTENT ? VSZ_BATCH_PERSISTENT : VSZ_BATCH_STAGED))
+                pick = c;
+        use_matcher(pick.sc);
+        schedule = pick.sched;
+        return VSZ_OK;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipEventCreate failed");
+    int rc = VSZ_OK;
+    float best_ms = -1.0f;
+    Cand best = cands[0];
+    for (const Cand &c : cands) {
+        use_matcher(c.sc);
+        schedule = c.sched;
+        rc = upload(noisy, dst, work, s);
+        float ms = -1.0f;
+        for (int rep = 0; rep < 3 && rc == VSZ_OK; ++rep) {
+            (void)hipEventRecord(e0, s);
+            rc = run_direct(work, s);
+            (void)hipEventRecord(e1, s);
+            if (rc == VSZ_OK && hipEventSynchronize(e1) != hipSuccess)
+                rc = vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: tuning run failed");
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            if (rep > 0 && (ms < 0.0f || t < ms)) ms = t;
+        }
+        if (rc) break;
+        if (best_ms < 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    use_matcher(best.sc);
+    schedule = best.sched;
+    return rc;
+}
+
 extern "C" {
 
 int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane *planes, int n, vsz_bm3d_batch **out) {
@@ -2759,31 +3073,34 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: need 1..%d planes", kMaxBatch);
     if (bp->t2d != 0 || bp->t1d != 0 || bp->transform_precision < 0 || bp->transform_precision > 1)
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: batches use the dct/dct transforms");
+    if (bp->schedule < VSZ_BATCH_STAGED || bp->schedule > VSZ_BATCH_AUTO)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: schedule must be staged, persistent or auto");
     auto b = std::make_unique<vsz_bm3d_batch>();
     b->bp = *bp;
     b->exact = vsz_exact_extractor(bp->extractor_exp);
     int ls = 0;
-    b->match_lds = match_lds_bytes(bp->block_step, bp->bm_range, &ls);
-    b->match_stride = ls;
-    if (!b->match_lds)
-        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: block_step/bm_range too large for the LDS-staged matcher");
-    // the shared-column matcher when every plane allows it (block_step 4, (W-8) % 4 == 0, ssd/sad)
-    b->shared_cols = bp->block_step == 4 && (bp->metric == 0 || bp->metric == 1);
-    for (int i = 0; i < n && b->shared_cols; ++i)
-        if ((planes[i].width - 8) % 4 != 0) b->shared_cols = false;
-    if (b->shared_cols) {
+    b->lds_rows = match_lds_bytes(bp->block_step, bp->bm_range, &ls);
+    b->stride_rows = ls;
+    // the shared-column matcher where every plane allows it (block_step 4, (W-8) % 4 == 0, ssd/sad)
+    b->sc_allowed = bp->block_step == 4 && (bp->metric == 0 || bp->metric == 1);
+    for (int i = 0; i < n && b->sc_allowed; ++i)
+        if ((planes[i].width - 8) % 4 != 0) b->sc_allowed = false;
+    if (b->sc_allowed) {
         MatchArgs probe{};
         probe.block_step = 4;
         probe.width = 8;
         probe.bm_range = bp->bm_range;
-        const size_t l4 = match_lds4_bytes(probe, &ls);
-        if (l4) {
-            b->match_lds = l4;
-            b->match_stride = ls;
-        } else {
-            b->shared_cols = false;
-        }
+        b->lds_sc = match_lds4_bytes(probe, &ls);
+        b->stride_sc = ls;
+        b->sc_allowed = b->lds_sc != 0;
     }
+    if (!b->lds_rows && !b->sc_allowed)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: block_step/bm_range too large for the LDS-staged matcher");
+    b->use_matcher(b->sc_allowed);
+    // the persistent form: exact aggregation, ssd or sad
+    b->persist_ok = b->exact && (bp->metric == 0 || bp->metric == 1);
+    if (bp->schedule == VSZ_BATCH_PERSISTENT && !b->persist_ok)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: the persistent schedule needs extractor_exp > 0 and the ssd or sad metric");
     const float fac[2] = {bits_float(0x3f021bb6u), bits_float(0x3e40c0c1u)};  // src/bm3d.zig:1013-1024
     size_t acc = 0, rest = 0;
     for (int i = 0; i < n; ++i) {
@@ -2811,15 +3128,21 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
             pl.tile[pass] = plan_tiles(&k, pl.L, 0);
             if (!pl.tile[pass].ok) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: no filter tile fits", i);
         }
-        pl.acc_off = acc;
+        pl.acc_off[0] = acc;
         acc += (pl.L.acc_bytes + 255) & ~size_t(255);
         b->planes.push_back(pl);
     }
-    b->acc_total = acc;
-    rest = acc;
+    // [pass 0 accumulators][pass 1 accumulators] then per plane [matches per pass][basic], every
+    // region 256-byte aligned (no cache line holds data of two regions)
+    b->acc_span = acc;
+    b->acc_total = acc * b->passes();
+    rest = b->acc_total;
     for (auto &pl : b->planes) {
-        pl.match_off = rest;
-        rest += pl.L.match_bytes;
+        pl.acc_off[1] = pl.acc_off[0] + b->acc_span;
+        for (int pass = 0; pass < b->passes(); ++pass) {
+            pl.match_off[pass] = rest;
+            rest += (pl.L.match_bytes + 255) & ~size_t(255);
+        }
         pl.basic_off = rest;
         rest += ((size_t)pl.L.ts * 4 + 255) & ~size_t(255);
     }
@@ -2830,6 +3153,20 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
         b->d_tab = nullptr;
         return vsz_fail(VSZ_ENOMEM, "vsz_bm3d_batch_create: hipMalloc failed");
     }
+    if (b->persist_ok) {
+        b->h_pt = new PersistTable();
+        if (hipMalloc(&b->d_pt, sizeof(PersistTable)) != hipSuccess) {
+            b->d_pt = nullptr;
+            return vsz_fail(VSZ_ENOMEM, "vsz_bm3d_batch_create: hipMalloc failed");
+        }
+        if (hipMalloc(&b->d_q, sizeof(BatchQueue)) != hipSuccess) {
+            b->d_q = nullptr;
+            return vsz_fail(VSZ_ENOMEM, "vsz_bm3d_batch_create: hipMalloc failed");
+        }
+        // the queue starts empty; every persistent launch leaves it empty again
+        if (hipMemset(b->d_q, 0, sizeof(BatchQueue)) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_create: hipMemset failed");
+    }
     if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
         b->cap = nullptr;
         return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_create: hipStreamCreate failed");
@@ -2838,7 +3175,7 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
         b->replayed = nullptr;
         return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_create: hipEventCreate failed");
     }
-    b->tuned = !bp->autotune;
+    b->tuned = false;  // the first run tunes (when asked) and picks the schedule
     *out = b.release();
     return VSZ_OK;
 }
@@ -2852,10 +3189,16 @@ int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *co
         if (!d_noisy[i] || !d_out[i]) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_run: null plane %zu", i);
     hipStream_t s = (hipStream_t)stream;
     char *work = static_cast<char *>(d_work);
+    if (b->launched && hipStreamWaitEvent(s, b->replayed, 0) != hipSuccess)  // runs never overlap
+        return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipStreamWaitEvent failed");
     if (!b->tuned) {
-        const int rc = b->tune(d_noisy, work, s);
+        if (b->launched && hipEventSynchronize(b->replayed) != hipSuccess)
+            return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: waiting for the previous run failed");
+        int rc = b->bp.autotune ? b->tune(d_noisy, work, s) : VSZ_OK;
+        if (rc == VSZ_OK) rc = b->tune_schedule(d_noisy, d_out, work, s);
         if (rc) return rc;
         b->tuned = true;
+        b->last_work = nullptr;  // the tables hold the last tried candidate's: upload again
     }
     const bool same = b->last_work == d_work && b->last_noisy.size() == n &&
                       std::equal(b->last_noisy.begin(), b->last_noisy.end(), d_noisy) &&
@@ -2864,12 +3207,8 @@ int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *co
         // an earlier replay, possibly on another stream, may still read the device tables
         if (b->launched && hipEventSynchronize(b->replayed) != hipSuccess)
             return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: waiting for the previous run failed");
-        b->fill_tables(d_noisy, d_out, work);
-        for (int pass = 0; pass < b->passes(); ++pass)
-            if (hipMemcpyAsync(b->d_tab + pass, b->h_tab[pass], sizeof(BatchTable), hipMemcpyHostToDevice, s) != hipSuccess)
-                return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
-        // the pageable source may be reused as soon as the copy is done
-        if (hipStreamSynchronize(s) != hipSuccess) return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: table upload failed");
+        const int rc = b->upload(d_noisy, d_out, work, s);
+        if (rc) return rc;
         b->last_work = d_work;
         b->last_noisy.assign(d_noisy, d_noisy + n);
         b->last_dst.assign(d_out, d_out + n);
@@ -2881,9 +3220,14 @@ int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *co
         }
     }
     if (!b->exec) {
+        if (b->schedule == VSZ_BATCH_PERSISTENT) {
+            const int rc = b->enqueue_persistent(nullptr, true);  // sizes the grid outside the capture
+            if (rc) return rc;
+        }
         if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipStreamBeginCapture failed");
-        b->enqueue(work, b->cap);
+        if (b->schedule == VSZ_BATCH_PERSISTENT) (void)b->enqueue_persistent(b->cap);
+        else b->enqueue(work, b->cap);
         if (hipStreamEndCapture(b->cap, &b->graph) != hipSuccess || !b->graph)
             return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: graph capture failed");
         if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
@@ -2902,6 +3246,24 @@ int vsz_bm3d_batch_tile(const vsz_bm3d_batch *b, int plane, int pass, int *tbx, 
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_tile: invalid arguments");
     *tbx = b->planes[plane].tile[pass].tbx;
     *tby = b->planes[plane].tile[pass].tby;
+    return VSZ_OK;
+}
+
+int vsz_bm3d_batch_check(vsz_bm3d_batch *b, void *stream) {
+    if (!b) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_check: invalid arguments");
+    if (!b->d_q) return VSZ_OK;
+    unsigned t = 0;
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess ||
+        hipMemcpy(&t, &b->d_q->timeouts, sizeof(t), hipMemcpyDeviceToHost) != hipSuccess)
+        return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_check: reading the queue failed");
+    if (t) return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_check: %u dependency waits of the persistent launch timed out", t);
+    return VSZ_OK;
+}
+
+int vsz_bm3d_batch_schedule(const vsz_bm3d_batch *b, int *schedule, int *shared_cols) {
+    if (!b || !schedule || !shared_cols) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_schedule: invalid arguments");
+    *schedule = b->schedule;
+    *shared_cols = b->shared_cols ? 1 : 0;
     return VSZ_OK;
 }
 

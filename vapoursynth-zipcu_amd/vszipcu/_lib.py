@@ -79,7 +79,7 @@ class BatchParams(C.Structure):
     _fields_ = [
         ("block_step", C.c_int), ("bm_range", C.c_int), ("extractor_exp", C.c_int),
         ("metric", C.c_int), ("t2d", C.c_int), ("t1d", C.c_int), ("transform_precision", C.c_int),
-        ("two_pass", C.c_int), ("autotune", C.c_int),
+        ("two_pass", C.c_int), ("autotune", C.c_int), ("schedule", C.c_int),
     ]
 
 
@@ -117,6 +117,8 @@ PROTOTYPES = {
     "vsz_bm3d_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
                                      C.c_void_p]),
     "vsz_bm3d_batch_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vsz_bm3d_batch_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vsz_bm3d_batch_check": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vsz_bm3d_batch_free": (None, [C.c_void_p]),
     "vsz_cnn_conv3x3": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

@@ -141,14 +141,17 @@ class Batch:
 
     planes: [(width, height, stride, sigma)] with the filter's raw sigma per plane."""
 
+    SCHEDULES = {"staged": 0, "persistent": 1, "auto": 2}
+
     def __init__(self, planes, *, block_step=4, bm_range=12, extractor_exp=3, metric="ssd", precision="fp32",
-                 two_pass=True, autotune=True):
+                 two_pass=True, autotune=True, schedule="auto"):
         bp = _lib.BatchParams()
         bp.block_step, bp.bm_range, bp.extractor_exp = block_step, bm_range, extractor_exp
         bp.metric = METRICS[metric.lower()] if isinstance(metric, str) else int(metric)
         bp.t2d = bp.t1d = 0
         bp.transform_precision = {"fp32": 0, "split-bf16": 1}[precision]
         bp.two_pass, bp.autotune = int(bool(two_pass)), int(bool(autotune))
+        bp.schedule = self.SCHEDULES[schedule]
         arr = (_lib.Plane * len(planes))()
         for i, (w, h, st, sg) in enumerate(planes):
             arr[i].width, arr[i].height, arr[i].stride, arr[i].sigma = w, h, st, sg
@@ -181,6 +184,16 @@ class Batch:
         ins = (C.c_void_p * n)(*[t.data_ptr() for t in noisy])
         outs = (C.c_void_p * n)(*[t.data_ptr() for t in out])
         _check(_lib.load().vsz_bm3d_batch_run(self._h, ins, outs, C.c_void_p(work.data_ptr()), _stream_handle(stream)))
+
+    def check(self, stream=None):
+        """Raises if a dependency wait of a persistent launch reached its time bound."""
+        _check(_lib.load().vsz_bm3d_batch_check(self._h, _stream_handle(stream)))
+
+    def schedule(self):
+        """(schedule name, shared-column matcher) the batch runs with; final after its first run."""
+        sch, sc = C.c_int(), C.c_int()
+        _check(_lib.load().vsz_bm3d_batch_schedule(self._h, C.byref(sch), C.byref(sc)))
+        return {0: "staged", 1: "persistent"}[sch.value], bool(sc.value)
 
     def tile(self, plane: int, pass_: int):
         tx, ty = C.c_int(), C.c_int()
