@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+
+#include <algorithm>
 #include <string.h>
 
 #include "vszipcu_bm3d.h"
@@ -38,87 +40,112 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 
 __device__ __forceinline__ float bf2f(unsigned short u) { return __uint_as_float((unsigned)u << 16); }
 
-// Stages the (TILE+2)^2 input halo of image n (zeros outside the image) into LDS. Every load of a
-// thread is issued before the first store, so the loads' latencies overlap.
-__device__ __forceinline__ void stage_halo(const unsigned short *x, int n, int H, int W, int x0, int y0, char *lds) {
-    constexpr int kParts = HALO * HALO * 4, kPer = (kParts + 255) / 256;
-    uint4 v[kPer];
+// The (TILE+2)^2 input halo of image n (zeros outside the image), in registers: 16 bytes per
+// thread and step, loads issued back to back so their latencies overlap.
+constexpr int kHaloParts = HALO * HALO * 4, kHaloPer = (kHaloParts + 255) / 256;
+
+__device__ __forceinline__ void halo_load(const unsigned short *x, int n, int H, int W, int x0, int y0,
+                                          uint4 (&v)[kHaloPer]) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
+    for (int k = 0; k < kHaloPer; ++k) {
         const int i = (int)threadIdx.x + 256 * k;
         const int pix = i >> 2, part = i & 3;
         const int hy = pix / HALO, hx = pix % HALO;
         const int gy = y0 + hy - 1, gx = x0 + hx - 1;
         v[k] = make_uint4(0, 0, 0, 0);
-        if (i < kParts && gy >= 0 && gy < H && gx >= 0 && gx < W)
+        if (i < kHaloParts && gy >= 0 && gy < H && gx >= 0 && gx < W)
             v[k] = *reinterpret_cast<const uint4 *>(x + (((long long)n * H + gy) * W + gx) * CH + part * 8);
     }
+}
+
+__device__ __forceinline__ void halo_store(char *lds, const uint4 (&v)[kHaloPer]) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
+    for (int k = 0; k < kHaloPer; ++k) {
         const int i = (int)threadIdx.x + 256 * k;
-        if (i < kParts) *reinterpret_cast<uint4 *>(lds + (i >> 2) * PIX_BYTES + (i & 3) * 16) = v[k];
+        if (i < kHaloParts) *reinterpret_cast<uint4 *>(lds + (i >> 2) * PIX_BYTES + (i & 3) * 16) = v[k];
     }
+}
+
+// Stages the halo of image n into LDS.
+__device__ __forceinline__ void stage_halo(const unsigned short *x, int n, int H, int W, int x0, int y0, char *lds) {
+    uint4 v[kHaloPer];
+    halo_load(x, n, H, W, x0, y0, v);
+    halo_store(lds, v);
 }
 
 // y = conv3x3(x) with the epilogue of `mode`:
 //   0: relu(acc + bias) -> bf16 [N][H][W][32]
 //   1: base[n][y][x] + acc[out 0] + bias[0] -> fp32 [N][H][ostride] (the residual output layer)
 //   2: acc * (aux[n][y][x][out] > 0) -> bf16 (backward data through the previous ReLU)
+// Persistent: workgroup g walks the 16x16 tiles g, g + G, ... (row-major over images) and loads
+// the next tile's halo into registers while it computes the current one from LDS; the weights
+// are loaded once. Wave w computes output channels hh*16 .. +15 (hh = w & 1) of rows
+// (w >> 1)*8 .. +7 of a tile.
 __global__ __launch_bounds__(256) void k_conv3x3(const unsigned short *__restrict__ x, const unsigned short *__restrict__ w,
-                                                 const float *__restrict__ bias, int H, int W, int mode,
+                                                 const float *__restrict__ bias, int N, int H, int W, int tiles, int mode,
                                                  const unsigned short *__restrict__ aux, const float *__restrict__ base,
                                                  void *__restrict__ y, int ostride) {
     __shared__ __attribute__((aligned(16))) char lds[HALO * HALO * PIX_BYTES];
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE, n = blockIdx.z;
+    const int tx_n = (W + TILE - 1) / TILE, ty_n = (H + TILE - 1) / TILE;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, g = lane >> 4;
-
-    // Wave w computes output channels hh*16 .. +15 (hh = w & 1) of rows (w >> 1)*8 .. +7: half the
-    // weights per lane, so more workgroups fit a CU and hide each other's staging latency.
     const int hh = wave & 1, r0 = (wave >> 1) * 8;
+    const bool active = !(mode == 1 && hh == 1);  // the residual output layer needs output channel 0 only
     bf16x8 wa[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) wa[t] = *reinterpret_cast<const bf16x8 *>(w + ((t * CH) + hh * 16 + j) * CH + g * 8);
+    const int o = hh * 16 + g * 4;
+    float bo[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (mode != 2)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bo[q] = bias[mode == 1 ? 0 : o + q];
 
-    stage_halo(x, n, H, W, x0, y0, lds);
-    __syncthreads();
-    if (mode == 1 && hh == 1) return;  // the residual output layer needs output channel 0 only
-
+    uint4 v[kHaloPer];
+    int tile = blockIdx.x;
+    if (tile < tiles) halo_load(x, tile / (tx_n * ty_n), H, W, (tile % tx_n) * TILE, ((tile / tx_n) % ty_n) * TILE, v);
+    for (; tile < tiles; tile += gridDim.x) {
+        const int n = tile / (tx_n * ty_n), x0 = (tile % tx_n) * TILE, y0 = ((tile / tx_n) % ty_n) * TILE;
+        __syncthreads();  // every wave is done with the previous tile's halo
+        halo_store(lds, v);
+        __syncthreads();
+        const int nt = tile + gridDim.x;
+        if (nt < tiles) halo_load(x, nt / (tx_n * ty_n), H, W, (nt % tx_n) * TILE, ((nt / tx_n) % ty_n) * TILE, v);
+        if (!active) continue;
 #pragma unroll 2
-    for (int rr = 0; rr < 8; ++rr) {
-        const int r = r0 + rr;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int rr = 0; rr < 8; ++rr) {
+            const int r = r0 + rr;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int ky = t / 3, kx = t % 3;
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(lds + ((r + ky) * HALO + j + kx) * PIX_BYTES + g * 16);
-            acc = mfma(wa[t], b, acc);
+            for (int t = 0; t < 9; ++t) {
+                const int ky = t / 3, kx = t % 3;
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(lds + ((r + ky) * HALO + j + kx) * PIX_BYTES + g * 16);
+                acc = mfma(wa[t], b, acc);
+            }
+            const int gy = y0 + r, gx = x0 + j;
+            if (gy >= H || gx >= W) continue;
+            const long long pix = ((long long)n * H + gy) * W + gx;
+            if (mode == 1) {
+                if (g == 0) static_cast<float *>(y)[((long long)n * H + gy) * ostride + gx] = base[pix] + (acc[0] + bo[0]);
+                continue;
+            }
+            float vv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) vv[q] = acc[q];
+            if (mode == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) vv[q] = fmaxf(vv[q] + bo[q], 0.0f);
+            } else {
+                const uint2 m = *reinterpret_cast<const uint2 *>(aux + pix * CH + o);
+                const unsigned short mm[4] = {(unsigned short)(m.x & 0xffff), (unsigned short)(m.x >> 16),
+                                              (unsigned short)(m.y & 0xffff), (unsigned short)(m.y >> 16)};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) vv[q] = bf2f(mm[q]) > 0.0f ? vv[q] : 0.0f;
+            }
+            bf16x4 o4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o4[q] = (__bf16)vv[q];
+            *reinterpret_cast<bf16x4 *>(static_cast<unsigned short *>(y) + pix * CH + o) = o4;
         }
-        const int gy = y0 + r, gx = x0 + j;
-        if (gy >= H || gx >= W) continue;
-        const long long pix = ((long long)n * H + gy) * W + gx;
-        if (mode == 1) {
-            if (g == 0) static_cast<float *>(y)[((long long)n * H + gy) * ostride + gx] = base[pix] + (acc[0] + bias[0]);
-            continue;
-        }
-        const int o = hh * 16 + g * 4;
-        float v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = acc[q];
-        if (mode == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = fmaxf(v[q] + bias[o + q], 0.0f);
-        } else {
-            const uint2 m = *reinterpret_cast<const uint2 *>(aux + pix * CH + o);
-            const unsigned short mm[4] = {(unsigned short)(m.x & 0xffff), (unsigned short)(m.x >> 16),
-                                          (unsigned short)(m.y & 0xffff), (unsigned short)(m.y >> 16)};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = bf2f(mm[q]) > 0.0f ? v[q] : 0.0f;
-        }
-        bf16x4 o4;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) o4[q] = (__bf16)v[q];
-        *reinterpret_cast<bf16x4 *>(static_cast<unsigned short *>(y) + pix * CH + o) = o4;
     }
 }
 
@@ -435,9 +462,16 @@ int vsz_cnn_conv3x3(const void *x, const void *w, const float *bias, int n, int 
     if (!x || !w || !y || n < 1 || h < 1 || wd < 1 || mode < 0 || mode > 2 || (mode != 2 && !bias) ||
         (mode == 2 && !aux) || (mode == 1 && (!base || out_stride < wd)) || n > 65535 || (h + TILE - 1) / TILE > 65535)
         return vsz_fail(VSZ_EINVAL, "vsz_cnn_conv3x3: invalid arguments");
-    const dim3 grid((wd + TILE - 1) / TILE, (h + TILE - 1) / TILE, n);
-    hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, (hipStream_t)stream, (const unsigned short *)x,
-                       (const unsigned short *)w, bias, h, wd, mode, (const unsigned short *)aux, base, y, out_stride);
+    const long long tiles = (long long)((wd + TILE - 1) / TILE) * ((h + TILE - 1) / TILE) * n;
+    if (tiles > (1LL << 30)) return vsz_fail(VSZ_EINVAL, "vsz_cnn_conv3x3: image too large");
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return vsz_fail(VSZ_EDEVICE, "vsz_cnn_conv3x3: no device");
+    // six workgroups per CU (their 26 KB halos fit the LDS together): measured best of 2..8
+    const int grid = (int)std::min<long long>(tiles, 6LL * std::max(cus, 1));
+    hipLaunchKernelGGL(k_conv3x3, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const unsigned short *)x,
+                       (const unsigned short *)w, bias, n, h, wd, (int)tiles, mode, (const unsigned short *)aux, base, y,
+                       out_stride);
     return launch_ok("vsz_cnn_conv3x3");
 }
 
