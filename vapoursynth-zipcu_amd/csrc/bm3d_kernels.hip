@@ -1464,8 +1464,12 @@ __device__ __forceinline__ void pair_forward(const SplitOp (&xa)[4], const PairC
 }
 
 // Inverse of pair_forward, including the factor 1/4096. zz[s] in the layout pair_forward
-// returns; out[t][e] = pixel Y[g = q/2][i = 2t + (l%16)/8][j = l%8][c = 4(q%2) + e], the
-// positions pair_load read.
+// returns. ROW_OUT: out[t][e] = pixel Y[g = q/2][i = 2t + (l%16)/8][j = l%8][c = 4(q%2) + e], the
+// positions pair_load read (four adjacent pixels of a row per lane, for the LDS region image);
+// otherwise I2 is a left product and out[t][e] = Y[g = (l%16)/8][i = 2t + q/2][j = 4(q%2) + e]
+// [c = l%8] (the 8 lanes of a row hold 8 adjacent pixels, so each device atomic instruction
+// updates whole row segments).
+template <bool ROW_OUT>
 __device__ __forceinline__ void pair_inverse(const vsz_f4 (&zz)[4], const PairConsts &K, float *tb, int l,
                                              vsz_f4 (&out)[4]) {
     const int q = l >> 4, h = (l & 15) >> 3, c = l & 7;
@@ -1483,7 +1487,8 @@ __device__ __forceinline__ void pair_inverse(const vsz_f4 (&zz)[4], const PairCo
         const vsz_f4 a = *reinterpret_cast<const vsz_f4 *>(tb + (q >> 1) * kPairSG + (2 * t + h) * kPairSI +
                                                             c * kPairSJ + 4 * (q & 1));
         const vsz_f4 v1 = mfma_split(split_f4(a), K.inv);    // I1: rows along c
-        out[t] = mfma_split(split_f4(v1), K.inv);            // I2: along j, on I1's result transposed
+        if constexpr (ROW_OUT) out[t] = mfma_split(split_f4(v1), K.inv);  // I2: along j, on I1's result transposed
+        else out[t] = mfma_split(K.inv, split_f4(v1));                    // I2: columns along j
     }
     lds_order();
 }
@@ -1555,10 +1560,10 @@ __device__ __forceinline__ float pair_shrink(vsz_f4 (&z)[4], const vsz_f4 (&r)[4
     }
 }
 
-// One group pair, loaded pixels (pair_load) to filtered pixels: returns out[t][e] = pixel
-// (g = q/2, i = 2t + (l%16)/8, j = l%8, c = 4(q%2) + e), the positions pair_load read, and the
-// weight of group q/2.
-template <bool FINAL>
+// One group pair, loaded pixels (pair_load) to filtered pixels out[t][e] in pair_inverse's
+// ROW_OUT layout; returns the weight of the group of the lane's output pixels (q/2 with ROW_OUT,
+// else (l%16)/8).
+template <bool FINAL, bool ROW_OUT>
 __device__ __forceinline__ float pair_filter(const vsz_f4 (&v)[4], const vsz_f4 (&vr)[4], float sigma,
                                              const PairConsts &K, float *tb, int l, vsz_f4 (&out)[4]) {
     SplitOp xv[4], xr[4];
@@ -1571,8 +1576,9 @@ __device__ __forceinline__ float pair_filter(const vsz_f4 (&v)[4], const vsz_f4 
     pair_forward(xv, K, tb, l, z);
     if constexpr (FINAL) pair_forward(xr, K, tb, l, r);
     const float w = pair_shrink<FINAL>(z, r, sigma, l);
-    pair_inverse(z, K, tb, l, out);
-    return w;
+    pair_inverse<ROW_OUT>(z, K, tb, l, out);
+    if constexpr (ROW_OUT) return w;
+    else return __shfl(w, ((l & 15) >> 3) * 32);
 }
 
 // Positions of a pair's patches in the lane's layout: (g = q/2, i = 2t + (l%16)/8).
@@ -1651,7 +1657,7 @@ __device__ __forceinline__ void filter_tile_body_mfma(const FilterArgs &a, int t
                 pair_load<FINAL>(a, p, cx, cy, zero4, l, cv, cr);
                 if (more) positions(base + 8, nx, ny);
                 vsz_f4 out[4];
-                const float w = pair_filter<FINAL>(cv, cr, a.sigma[p], K, tb, l, out);
+                const float w = pair_filter<FINAL, true>(cv, cr, a.sigma[p], K, tb, l, out);
                 if (base + (q >> 1) < ngroups) {  // else the pair's second group is past the tile
                     const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
                     const float wq = (w + E) - E;
@@ -1724,34 +1730,40 @@ __device__ __forceinline__ void filter_tile_body(const FilterArgs &a, int tile_x
 template <bool FINAL, bool EXACT>
 __global__ __launch_bounds__(64) void k_filter_mfma(FilterArgs a) {
     __shared__ __align__(16) float tbuf[kPairBufFloats];
-    const int l = threadIdx.x, q = l >> 4, j = l & 7, c0 = 4 * (q & 1);
+    const int l = threadIdx.x, q = l >> 4, h = (l & 15) >> 3, c = l & 7;
     const long long gid0 = min((long long)blockIdx.x * 2, a.nblocks - 1);
     const long long gid1 = min((long long)blockIdx.x * 2 + 1, a.nblocks - 1);
-    const bool valid = (long long)blockIdx.x * 2 + (q >> 1) < a.nblocks;
+    const bool valid = (long long)blockIdx.x * 2 + h < a.nblocks;  // the output group is (l%16)/8
     int px[4], py[4], pz[4];
     pair_positions(a.matches + gid0 * 8, a.matches + gid1 * 8, l, px, py, pz);
+    int4 po[4];  // output patches: (g = (l%16)/8, i = 2t + q/2)
+    {
+        const int4 *mb = h ? a.matches + gid1 * 8 : a.matches + gid0 * 8;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) po[t] = mb[2 * t + (q >> 1)];
+    }
     const PairConsts K = pair_consts(l);
     const float E = a.extractor;
     for (int p = 0; p < a.num_planes; ++p) {
         if (!((a.active_mask >> p) & 1)) continue;
         vsz_f4 v[4], vr[4], out[4];
         pair_load<FINAL>(a, p, px, py, pz, l, v, vr);
-        const float w = pair_filter<FINAL>(v, vr, a.sigma[p], K, tbuf, l, out);
+        const float w = pair_filter<FINAL, false>(v, vr, a.sigma[p], K, tbuf, l, out);
         if (!valid) continue;
         const long long pb = (long long)p * a.acc_plane_elems;
         const float ws = __builtin_ldexpf(w, 24 - a.extractor_exp);
         const float wq = (w + E) - E;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const long long pbase = pb + (long long)pz[t] * 2 * a.ts + (long long)py[t] * a.stride + px[t];
-            const long long at0 = pbase + (long long)j * a.stride + c0;
+            const long long pbase = pb + (long long)po[t].z * 2 * a.ts + (long long)po[t].y * a.stride + po[t].x;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
+                const long long at = pbase + (long long)(4 * (q & 1) + e) * a.stride + c;
                 if constexpr (EXACT)
-                    atomicAdd((unsigned long long *)a.acc + at0 + e, (unsigned long long)fixed_from_scaled(scaled_term(ws, out[t][e])));
-                else atomicAdd((float *)a.acc + at0 + e, (w * out[t][e] + E) - E);
+                    atomicAdd((unsigned long long *)a.acc + at, (unsigned long long)fixed_from_scaled(scaled_term(ws, out[t][e])));
+                else atomicAdd((float *)a.acc + at, (w * out[t][e] + E) - E);
             }
-            if (j == 0 && (q & 1) == 0) {
+            if (c == 0 && (q & 1) == 0) {
                 if constexpr (EXACT)
                     atomicAdd((unsigned long long *)a.acc + pbase + a.ts, (unsigned long long)to_fixed(wq, a.extractor_exp));
                 else atomicAdd((float *)a.acc + pbase + a.ts, wq);
