@@ -1938,10 +1938,13 @@ __global__ __launch_bounds__(256) void k_finish_batch(const BatchTable *__restri
 //   pass 1 match and zeroing <- pass 0 finishing of the plane (its basic estimate, its
 //   accumulator). Those items always come earlier in the order, so the earliest unfinished item can
 // run and the launch drains with any number of resident workgroups; a plane's stages overlap with
-// other planes' earlier stages. Completion is published with agent-scope release (each thread,
-// then the counter update) and observed with agent-scope acquire, which covers the per-XCD L2s.
+// other planes' earlier stages. Completion: every wave waits for its own writes before the
+// item's closing barrier, then lane 0 publishes with an agent-scope release increment (which writes
+// back the XCD's L2); a wait polls with relaxed coherent loads and lane 0 acquires once (its CU's
+// caches serve the whole workgroup), so neither polling nor the other waves invalidate caches.
 enum : int { kStageZero = 0, kStageMatch = 1, kStageFilter = 2, kStageFinish = 3 };
-constexpr int kZeroChunk = 256 * 16;  // 64-bit words per zeroing item
+constexpr int kZeroChunk = 256 * 128;  // 64-bit words per zeroing item (256 KB)
+constexpr int kFinishPer = 8;          // finishing tiles (64 x 16 pixels) per item
 constexpr int kMaxSegs = 2 * 4 * kMaxBatch;
 constexpr int kSpinLimit = 1 << 24;  // polls of 256 cycles: a bound far above any legitimate wait
 
@@ -1964,9 +1967,10 @@ struct BatchQueue {
     unsigned done[2][4][kMaxBatch];
 };
 
+// Polls with relaxed coherent loads (no cache invalidation per poll); the caller acquires once.
 __device__ __forceinline__ void await_count(const unsigned *cnt, int need, unsigned *timeouts) {
     for (int spin = 0; spin < kSpinLimit; ++spin) {
-        if (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)need) return;
+        if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (unsigned)need) return;
         __builtin_amdgcn_s_sleep(4);
     }
     atomicAdd(timeouts, 1u);
@@ -1979,7 +1983,7 @@ __device__ __forceinline__ void await_count(const unsigned *cnt, int need, unsig
 // joined the two lane-0 blocks across the back edge into an outer loop of their own, and the
 // other lanes re-entered the item loop without a new claim.)
 template <int M, bool SC, int T>
-__global__ __launch_bounds__(256) void k_batch_persistent(const BatchTable *__restrict__ tabs,
+__global__ __launch_bounds__(256, 2) void k_batch_persistent(const BatchTable *__restrict__ tabs,
                                                           const PersistTable *__restrict__ pt, BatchQueue *q) {
     // one dynamic LDS block: 16 bytes of loop control, then the stages' 16-byte aligned memory
     extern __shared__ __align__(16) float lds[];
@@ -2017,9 +2021,10 @@ __global__ __launch_bounds__(256) void k_batch_persistent(const BatchTable *__re
             } else if (pass == 1) {
                 await_count(&q->done[0][kStageFinish][p], pt->count[0][kStageFinish][p], &q->timeouts);
             }
+            // one acquire per workgroup: its waves share this CU's caches and wait at the barrier
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const BatchTable &t = tabs[pass];
         if (stage == kStageZero) {
             unsigned long long *z = pt->acc[pass][p];
@@ -2037,9 +2042,17 @@ __global__ __launch_bounds__(256) void k_batch_persistent(const BatchTable *__re
             long long (*sD)[72] = reinterpret_cast<long long (*)[72]>(smem);
             long long (*sH)[64] = reinterpret_cast<long long (*)[64]>(smem + 23 * 72 * 2);
             const int tx = t.finish_tiles_x[p], ty = t.finish_tiles_y[p];
-            finish_body<true>(t.fin[p], local % tx, (local / tx) % ty, local / (tx * ty), sD, sH);
+            const int nfin = t.finish_off[p + 1] - t.finish_off[p];
+            for (int k = 0; k < kFinishPer; ++k) {
+                const int ft = local * kFinishPer + k;
+                if (ft >= nfin) break;  // uniform: every wave leaves together
+                if (k > 0) __syncthreads();  // the previous tile's reads of sD / sH are done
+                finish_body<true>(t.fin[p], ft % tx, (ft / tx) % ty, ft / (tx * ty), sD, sH);
+            }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // every wave's writes have left it before the barrier; lane 0 then releases them for the
+        // workgroup (cache write-back and the counter update) when it publishes the item
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         prev_pass = pass;
         prev_stage = stage;
@@ -2884,7 +2897,7 @@ struct vsz_bm3d_batch {
                     if (stage == kStageZero) n = (int)((planes[i].L.acc_bytes / 8 + kZeroChunk - 1) / kZeroChunk);
                     else if (stage == kStageMatch) n = bt.match_off[i + 1] - bt.match_off[i];
                     else if (stage == kStageFilter) n = bt.filter_off[i + 1] - bt.filter_off[i];
-                    else n = bt.finish_off[i + 1] - bt.finish_off[i];
+                    else n = (bt.finish_off[i + 1] - bt.finish_off[i] + kFinishPer - 1) / kFinishPer;
                     t.count[pass][stage][i] = n;
                     if (n == 0) continue;
                     t.seg[t.nseg++] = BatchSeg{item, pass, stage, i};
