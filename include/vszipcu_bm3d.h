@@ -122,7 +122,8 @@ typedef struct vsz_bm3d_plane {
 
 typedef struct vsz_bm3d_batch_params {
     int block_step, bm_range, extractor_exp;
-    int metric, t2d, t1d;        /* t2d = t1d = 0 (dct) */
+    int metric, t2d, t1d;        /* one transform for both stages: t2d == t1d (dct 0, haar 1, wht 2,
+                                    bior1.5 3); dct with split-BF16 and the persistent schedule */
     int transform_precision;     /* as in vsz_bm3d_kparams */
     int two_pass;                /* 1: basic, then Wiener with ref = basic; 0: basic only */
     int autotune;                /* 1: on the first run, time every fitting filter tile shape per

@@ -29,10 +29,14 @@ def test_batch_rejects_bad_requests():
         device.Batch([(64, 48, 60, 5.0)], **KW)          # stride < width
     with pytest.raises(RuntimeError, match="too large for the LDS-staged matcher"):
         device.Batch([(640, 480, 640, 5.0)], block_step=8, bm_range=120)
-    with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0 and the ssd or sad"):
+    with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0, the ssd or sad"):
         device.Batch([(64, 48, 64, 5.0)], schedule="persistent", metric="zssd", **KW)
-    with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0 and the ssd or sad"):
+    with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0, the ssd or sad"):
         device.Batch([(64, 48, 64, 5.0)], schedule="persistent", block_step=4, bm_range=12, extractor_exp=0)
+    with pytest.raises(RuntimeError, match="persistent schedule needs .* the dct transform"):
+        device.Batch([(64, 48, 64, 5.0)], schedule="persistent", transform="haar", **KW)
+    with pytest.raises(RuntimeError, match="one transform for both stages"):
+        device.Batch([(64, 48, 64, 5.0)], transform="wht", precision="split-bf16", **KW)
 
 
 def _inputs(torch, planes, seed):
@@ -86,6 +90,38 @@ def test_batch_equals_single_plane_runs(torch_gpu, planes, autotune, precision, 
         assert not sc                 # a width with (W - 8) % 4 != 0 rules the shared-column matcher out
     elif not autotune:
         assert sc                     # preferred where allowed
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("transform", ["haar", "wht", "bior1.5"])
+@pytest.mark.parametrize("exact", [True, False])
+def test_batch_other_transforms_equal_single_plane_runs(torch_gpu, transform, exact):
+    """A batch with one transform for both stages against each plane run alone with that transform
+    (extractor 3: bit-identical; extractor 0: FP32 atomics in any order, so within rounding)."""
+    torch = torch_gpu
+    kw = dict(block_step=4, bm_range=12, extractor_exp=3 if exact else 0)
+    print(f"batch: transform={transform} exact={exact}", flush=True)
+    b = device.Batch(PLANES, autotune=True, transform=transform, **kw)
+    noisy = _inputs(torch, PLANES, 70)
+    outs = [torch.full((h, st), -1.0, dtype=torch.float32, device="cuda") for (w, h, st, _) in PLANES]
+    work = torch.zeros(b.workspace_size, dtype=torch.uint8, device="cuda")
+    b.run(noisy, outs, work)
+    torch.cuda.synchronize()
+    for i, (w, h, st, sg) in enumerate(PLANES):
+        kb = device.kparams(w, h, st, sigma=device.scaled_sigma([sg] * 3, False), t2d=transform, t1d=transform, **kw)
+        kf = device.kparams(w, h, st, sigma=device.scaled_sigma([sg] * 3, True), final=True, t2d=transform,
+                            t1d=transform, **kw)
+        wk = torch.zeros(max(device.workspace_size(kb), device.workspace_size(kf)), dtype=torch.uint8, device="cuda")
+        basic = torch.zeros((1, h, st), dtype=torch.float32, device="cuda")
+        want = torch.zeros((1, h, st), dtype=torch.float32, device="cuda")
+        device.run(kb, noisy[i][None, None], None, wk, basic, 0)
+        device.run(kf, basic[:, None], noisy[i][None, None], wk, want, 0)
+        torch.cuda.synchronize()
+        got, ref = outs[i].cpu().numpy()[:, :w], want[0].cpu().numpy()[:, :w]
+        if exact:
+            assert np.array_equal(got, ref), f"plane {i}"
+        else:
+            assert np.abs(got - ref).max() <= 1e-5, f"plane {i}: {np.abs(got - ref).max()}"
 
 
 @pytest.mark.gpu

@@ -2630,6 +2630,16 @@ float bits_float(uint32_t u) {
     return f;
 }
 
+// One tiled filter launch of the uniform transform `kind` (both stages), for the autotuner.
+void launch_filter_tile_kind(bool final_pass, int kind, const FilterArgs &a, const TilePlan &tp, hipStream_t s) {
+    switch (kind) {
+    case 0: final_pass ? launch_filter_3<true, 0, 0>(a, tp, s) : launch_filter_3<false, 0, 0>(a, tp, s); break;
+    case 1: final_pass ? launch_filter_3<true, 1, 1>(a, tp, s) : launch_filter_3<false, 1, 1>(a, tp, s); break;
+    case 2: final_pass ? launch_filter_3<true, 2, 2>(a, tp, s) : launch_filter_3<false, 2, 2>(a, tp, s); break;
+    default: final_pass ? launch_filter_3<true, 3, 3>(a, tp, s) : launch_filter_3<false, 3, 3>(a, tp, s); break;
+    }
+}
+
 // Autotuner results, shared by every plan of the process: (shape, pass, kernel variant) -> tile.
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int, int, int, int, int>, std::pair<int, int>> g_tune;
@@ -2637,6 +2647,18 @@ std::map<std::tuple<int, int, int, int, int, int, int, int>, std::pair<int, int>
 template <bool FINAL, int T, bool EXACT>
 void launch_filter_batch(const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_filter_tile_batch<FINAL, T, T, EXACT>), dim3(wgs), dim3(256), lds, s, d);
+}
+
+// kind: the batch's transform (dct 0, haar 1, wht 2, bior1.5 3, both stages) or kDctMfma.
+template <bool FINAL, bool EXACT>
+void launch_filter_batch_kind(int kind, const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
+    switch (kind) {
+    case 0: launch_filter_batch<FINAL, 0, EXACT>(d, wgs, lds, s); break;
+    case 1: launch_filter_batch<FINAL, 1, EXACT>(d, wgs, lds, s); break;
+    case 2: launch_filter_batch<FINAL, 2, EXACT>(d, wgs, lds, s); break;
+    case 3: launch_filter_batch<FINAL, 3, EXACT>(d, wgs, lds, s); break;
+    default: launch_filter_batch<FINAL, kDctMfma, EXACT>(d, wgs, lds, s); break;
+    }
 }
 
 void launch_match_batch(int metric, bool shared_cols, const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
@@ -2850,12 +2872,13 @@ struct vsz_bm3d_batch {
             const int fw = filter_wgs(pass);
             const size_t fl = filter_lds(pass);
             const bool mf = bp.transform_precision == 1;
+            const int kind = mf ? kDctMfma : bp.t2d;
             if (pass == 0) {
-                if (mf) exact ? launch_filter_batch<false, kDctMfma, true>(d, fw, fl, s) : launch_filter_batch<false, kDctMfma, false>(d, fw, fl, s);
-                else exact ? launch_filter_batch<false, 0, true>(d, fw, fl, s) : launch_filter_batch<false, 0, false>(d, fw, fl, s);
+                if (exact) launch_filter_batch_kind<false, true>(kind, d, fw, fl, s);
+                else launch_filter_batch_kind<false, false>(kind, d, fw, fl, s);
             } else {
-                if (mf) exact ? launch_filter_batch<true, kDctMfma, true>(d, fw, fl, s) : launch_filter_batch<true, kDctMfma, false>(d, fw, fl, s);
-                else exact ? launch_filter_batch<true, 0, true>(d, fw, fl, s) : launch_filter_batch<true, 0, false>(d, fw, fl, s);
+                if (exact) launch_filter_batch_kind<true, true>(kind, d, fw, fl, s);
+                else launch_filter_batch_kind<true, false>(kind, d, fw, fl, s);
             }
             if (exact) hipLaunchKernelGGL(k_finish_batch<true>, dim3(finish_wgs()), dim3(256), 0, s, d);
             else hipLaunchKernelGGL(k_finish_batch<false>, dim3(finish_wgs()), dim3(256), 0, s, d);
@@ -2957,7 +2980,7 @@ int vsz_bm3d_batch::tune(const float *const *noisy, char *work, hipStream_t s) {
         for (int pass = 0; pass < passes() && rc == VSZ_OK; ++pass) {
             const vsz_bm3d_kparams &k = pl.k[pass];
             const auto key = std::make_tuple(k.width, k.height, k.block_step, k.bm_range, pass, bp.metric,
-                                             bp.transform_precision, exact ? 1 : 0);
+                                             bp.transform_precision * 4 + bp.t2d, exact ? 1 : 0);  // kernel variant
             {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
                 auto it = g_tune.find(key);
@@ -2999,16 +3022,15 @@ int vsz_bm3d_batch::tune(const float *const *noisy, char *work, hipStream_t s) {
                 f.tbx = c.tbx;
                 f.tby = c.tby;
                 f.guard_bits = c.guard_bits;
-                const int T = bp.transform_precision == 1 ? kDctMfma : 0;
+                const bool mf = bp.transform_precision == 1;
                 float ms = 0.0f;
                 for (int rep = 0; rep < 3 && rc == VSZ_OK; ++rep) {
                     (void)hipEventRecord(e0, s);
-                    if (T == kDctMfma) {
+                    if (mf) {
                         if (pass == 0) launch_filter_3<false, kDctMfma, kDctMfma>(f, c, s);
                         else launch_filter_3<true, kDctMfma, kDctMfma>(f, c, s);
                     } else {
-                        if (pass == 0) launch_filter_3<false, 0, 0>(f, c, s);
-                        else launch_filter_3<true, 0, 0>(f, c, s);
+                        launch_filter_tile_kind(pass == 1, bp.t2d, f, c, s);
                     }
                     (void)hipEventRecord(e1, s);
                     if (hipEventSynchronize(e1) != hipSuccess) rc = vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: tuning launch failed");
@@ -3096,8 +3118,10 @@ extern "C" {
 int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane *planes, int n, vsz_bm3d_batch **out) {
     if (!bp || !planes || !out || n < 1 || n > kMaxBatch)
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: need 1..%d planes", kMaxBatch);
-    if (bp->t2d != 0 || bp->t1d != 0 || bp->transform_precision < 0 || bp->transform_precision > 1)
-        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: batches use the dct/dct transforms");
+    if (bp->t2d < 0 || bp->t2d > 3 || bp->t1d != bp->t2d || bp->transform_precision < 0 || bp->transform_precision > 1 ||
+        (bp->transform_precision == 1 && bp->t2d != 0))
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: a batch uses one transform for both stages (t2d == t1d; "
+                                    "dct for split-BF16)");
     if (bp->schedule < VSZ_BATCH_STAGED || bp->schedule > VSZ_BATCH_AUTO)
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: schedule must be staged, persistent or auto");
     auto b = std::make_unique<vsz_bm3d_batch>();
@@ -3123,9 +3147,10 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
         return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: block_step/bm_range too large for the LDS-staged matcher");
     b->use_matcher(b->sc_allowed);
     // the persistent form: exact aggregation, ssd or sad
-    b->persist_ok = b->exact && (bp->metric == 0 || bp->metric == 1);
+    b->persist_ok = b->exact && (bp->metric == 0 || bp->metric == 1) && bp->t2d == 0;
     if (bp->schedule == VSZ_BATCH_PERSISTENT && !b->persist_ok)
-        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: the persistent schedule needs extractor_exp > 0 and the ssd or sad metric");
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: the persistent schedule needs extractor_exp > 0, the ssd or sad "
+                                    "metric and the dct transform");
     const float fac[2] = {bits_float(0x3f021bb6u), bits_float(0x3e40c0c1u)};  // src/bm3d.zig:1013-1024
     size_t acc = 0, rest = 0;
     for (int i = 0; i < n; ++i) {
@@ -3146,6 +3171,7 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
             k.sigma[0] = k.sigma[1] = k.sigma[2] = planes[i].sigma * fac[pass];
             k.extractor_exp = bp->extractor_exp;
             k.metric = bp->metric;
+            k.t2d = k.t1d = bp->t2d;
             k.proc_mask = 1;
             k.transform_precision = bp->transform_precision;
             if (!kparams_ok(&k)) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: invalid geometry", i);

@@ -144,11 +144,11 @@ class Batch:
     SCHEDULES = {"staged": 0, "persistent": 1, "auto": 2}
 
     def __init__(self, planes, *, block_step=4, bm_range=12, extractor_exp=3, metric="ssd", precision="fp32",
-                 two_pass=True, autotune=True, schedule="auto"):
+                 two_pass=True, autotune=True, schedule="auto", transform="dct"):
         bp = _lib.BatchParams()
         bp.block_step, bp.bm_range, bp.extractor_exp = block_step, bm_range, extractor_exp
         bp.metric = METRICS[metric.lower()] if isinstance(metric, str) else int(metric)
-        bp.t2d = bp.t1d = 0
+        bp.t2d = bp.t1d = TRANSFORMS[transform.lower()] if isinstance(transform, str) else int(transform)
         bp.transform_precision = {"fp32": 0, "split-bf16": 1}[precision]
         bp.two_pass, bp.autotune = int(bool(two_pass)), int(bool(autotune))
         bp.schedule = self.SCHEDULES[schedule]
