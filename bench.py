@@ -262,7 +262,7 @@ def bench_batch(args):
     torch.cuda.synchronize()
     dt_seq = (time.perf_counter() - t0) / max(1, args.steps // 2)
     tiles = sorted({b.tile(i, 0) for i in range(64)} | {b.tile(i, 1) for i in range(64)})
-    chosen, shared_cols = b.schedule()
+    chosen, matcher = b.schedule()
     sched_ms = {}
     for sched in ("staged", "persistent"):
         bs_ = device.Batch(planes, block_step=4, bm_range=12, extractor_exp=3, autotune=True, schedule=sched)
@@ -286,7 +286,7 @@ def bench_batch(args):
                       "config": {"workload": "64 Gray planes, sizes {320..3840}x{240..2160} (seed 4), sigma U[1,20], "
                                              "two-pass, bs 4, R 12, extractor 3, one hipGraph per run",
                                  "mpix": round(px / 1e6, 2), "schedule": chosen,
-                                 "matcher": "shared-column" if shared_cols else "rows",
+                                 "matcher": matcher,
                                  "staged_ms": sched_ms["staged"], "persistent_ms": sched_ms["persistent"],
                                  "sequential_ms": round(dt_seq * 1e3, 3),
                                  "sequential_mpix_s": round(px / dt_seq / 1e6, 3),

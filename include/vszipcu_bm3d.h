@@ -154,8 +154,9 @@ int vsz_bm3d_batch_run(vsz_bm3d_batch *b, const float *const *d_noisy, float *co
 /* Filter tile (in reference blocks) chosen for plane i, pass 0 (basic) or 1 (final). */
 int vsz_bm3d_batch_tile(const vsz_bm3d_batch *b, int plane, int pass, int *tbx, int *tby);
 /* Schedule (VSZ_BATCH_STAGED or VSZ_BATCH_PERSISTENT) and matcher (1: shared-column block_step-4
- * matcher, 0: row matcher) of the batch; final after its first run. */
-int vsz_bm3d_batch_schedule(const vsz_bm3d_batch *b, int *schedule, int *shared_cols);
+ * matcher, 0: row matcher, 2: global-memory matcher for windows beyond the LDS budget) of the
+ * batch; final after its first run. */
+int vsz_bm3d_batch_schedule(const vsz_bm3d_batch *b, int *schedule, int *matcher);
 /* Waits for the work queued on `stream`, then reports whether any dependency wait of the batch's
  * persistent launches (since its creation) ran into its time bound, i.e. ran before its inputs
  * were complete: VSZ_OK, or an error. Always VSZ_OK for the staged schedule. */

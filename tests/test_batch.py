@@ -27,10 +27,12 @@ def test_batch_rejects_bad_requests():
         device.Batch([(64, 48, 64, 0.0)], **KW)
     with pytest.raises(RuntimeError, match="invalid geometry"):
         device.Batch([(64, 48, 60, 5.0)], **KW)          # stride < width
-    with pytest.raises(RuntimeError, match="too large for the LDS-staged matcher"):
-        device.Batch([(640, 480, 640, 5.0)], block_step=8, bm_range=120)
     with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0, the ssd or sad"):
         device.Batch([(64, 48, 64, 5.0)], schedule="persistent", metric="zssd", **KW)
+    with pytest.raises(RuntimeError, match="windows within the LDS budgets"):
+        device.Batch([(640, 480, 640, 5.0)], schedule="persistent", block_step=8, bm_range=120, extractor_exp=3)
+    with pytest.raises(RuntimeError, match="split-BF16 batches need LDS tiles"):
+        device.Batch([(640, 480, 640, 5.0)], precision="split-bf16", block_step=8, bm_range=120, extractor_exp=3)
     with pytest.raises(RuntimeError, match="persistent schedule needs extractor_exp > 0, the ssd or sad"):
         device.Batch([(64, 48, 64, 5.0)], schedule="persistent", block_step=4, bm_range=12, extractor_exp=0)
     with pytest.raises(RuntimeError, match="persistent schedule needs .* the dct transform"):
@@ -83,13 +85,41 @@ def test_batch_equals_single_plane_runs(torch_gpu, planes, autotune, precision, 
         assert np.array_equal(got, want), f"plane {i} {p}"
         tx, ty = b.tile(i, 0)
         assert tx * ty >= 1
-    sched, sc = b.schedule()
+    sched, matcher = b.schedule()
     if schedule != "auto":
         assert sched == schedule
     if planes == "uneven":
-        assert not sc                 # a width with (W - 8) % 4 != 0 rules the shared-column matcher out
+        assert matcher == "rows"      # a width with (W - 8) % 4 != 0 rules the shared-column matcher out
     elif not autotune:
-        assert sc                     # preferred where allowed
+        assert matcher == "shared-column"   # preferred where allowed
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bm_range", [48, 120])
+def test_batch_beyond_lds_budgets_equals_single_plane_runs(torch_gpu, bm_range):
+    """Windows whose staged matcher region exceeds the LDS budget take the global-memory matcher;
+    at R 120 the 640 x 480 plane's filter region exceeds the tile budget too (direct atomics)."""
+    torch = torch_gpu
+    P = [(640, 480, 640, 6.0), (96, 64, 96, 9.0), (200, 120, 208, 3.0)]
+    kw = dict(block_step=8, bm_range=bm_range, extractor_exp=3)
+    print(f"batch beyond LDS: R={bm_range}", flush=True)
+    b = device.Batch(P, autotune=True, **kw)
+    noisy = _inputs(torch, P, 90)
+    outs = [torch.full((h, st), -1.0, dtype=torch.float32, device="cuda") for (w, h, st, _) in P]
+    work = torch.zeros(b.workspace_size, dtype=torch.uint8, device="cuda")
+    b.run(noisy, outs, work)
+    torch.cuda.synchronize()
+    assert b.schedule() == ("staged", "global")
+    for i, (w, h, st, sg) in enumerate(P):
+        kb = device.kparams(w, h, st, sigma=device.scaled_sigma([sg] * 3, False), **kw)
+        kf = device.kparams(w, h, st, sigma=device.scaled_sigma([sg] * 3, True), final=True, **kw)
+        wk = torch.zeros(max(device.workspace_size(kb), device.workspace_size(kf)), dtype=torch.uint8, device="cuda")
+        basic = torch.zeros((1, h, st), dtype=torch.float32, device="cuda")
+        want = torch.zeros((1, h, st), dtype=torch.float32, device="cuda")
+        device.run(kb, noisy[i][None, None], None, wk, basic, 0)
+        device.run(kf, basic[:, None], noisy[i][None, None], wk, want, 0)
+        torch.cuda.synchronize()
+        assert np.array_equal(outs[i].cpu().numpy()[:, :w], want[0].cpu().numpy()[:, :w]), f"plane {i}"
 
 
 @pytest.mark.gpu

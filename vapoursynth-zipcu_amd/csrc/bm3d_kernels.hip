@@ -833,9 +833,7 @@ __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
 // Spatial search on raw buffer loads, for windows whose staged region exceeds the LDS budget: one
 // lane per reference block, 8 x 8 blocks per wave.
 template <int M>
-__global__ __launch_bounds__(64) void k_match(MatchArgs a) {
-    const int lane = threadIdx.x;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void match_global_body(const MatchArgs &a, int tile, int lane) {
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
     const int bx = tx * 8 + (lane & 7), by = ty * 8 + (lane >> 3);
     const bool valid = bx < a.nbx && by < a.nby;
@@ -874,6 +872,11 @@ __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
     int4 *out = a.matches + ((long long)by * a.nbx + bx) * 8;
 #pragma unroll
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
+}
+
+template <int M>
+__global__ __launch_bounds__(64) void k_match(MatchArgs a) {
+    match_global_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
 }
 
 // Temporal matching after the centre frame's spatial search: reads the centre lists, runs the
@@ -1228,10 +1231,9 @@ __device__ __forceinline__ void scatter_den(const FilterArgs &a, long long pb, c
 // Direct path: every numerator contribution is a device-wide atomic (temporal passes and
 // search windows too large for an LDS tile).
 template <bool FINAL, int T2, int T1, bool EXACT>
-__global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
-    __shared__ float tbuf[8][72];
-    const int lane = threadIdx.x, g = lane >> 3, c = lane & 7;
-    const int gid = blockIdx.x * 8 + g;
+__device__ __forceinline__ void filter_direct_body(const FilterArgs &a, long long gbase, int lane, float (*tbuf)[72]) {
+    const int g = lane >> 3, c = lane & 7;
+    const long long gid = gbase + g;
     const bool valid = gid < a.nblocks;
     int mx[8], my[8], mz[8];
     load_matches(a.matches + (long long)(valid ? gid : 0) * 8, mx, my, mz);
@@ -1262,6 +1264,12 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
         }
         if (c == 0) scatter_den<EXACT>(a, pb, mx, my, mz, (w + E) - E);
     }
+}
+
+template <bool FINAL, int T2, int T1, bool EXACT>
+__global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
+    __shared__ float tbuf[8][72];
+    filter_direct_body<FINAL, T2, T1, EXACT>(a, (long long)blockIdx.x * 8, threadIdx.x, tbuf);
 }
 
 // Adds a tile's LDS region image (rw x rh cells at rx0, ry0) to one plane's device accumulator
@@ -1917,6 +1925,27 @@ __global__ __launch_bounds__(256, 2) void k_filter_tile_batch(const BatchTable *
     const FilterArgs &a = t->f[p];
     const int local = g - t->filter_off[p], tx = t->filter_tiles_x[p];
     filter_tile_body<FINAL, T2, T1, EXACT>(a, local % tx, local / tx, smem);
+}
+
+// Windows beyond the LDS budgets: the global-memory matcher (each wave one 8 x 8-block tile) and
+// the direct-atomic filter (each wave 8 groups), over a batch.
+template <int M>
+__global__ __launch_bounds__(256) void k_match_global_batch(const BatchTable *__restrict__ t) {
+    const int g = blockIdx.x;
+    const int p = batch_plane(t->match_off, t->n, g);
+    const MatchArgs &a = t->m[p];
+    const int tile = (g - t->match_off[p]) * 4 + (threadIdx.x >> 6);
+    if (tile >= a.tiles_x * ((a.nby + 7) / 8)) return;
+    match_global_body<M>(a, tile, threadIdx.x & 63);
+}
+
+template <bool FINAL, int T2, int T1, bool EXACT>
+__global__ __launch_bounds__(256) void k_filter_direct_batch(const BatchTable *__restrict__ t) {
+    __shared__ float tbuf[4][8][72];
+    const int g = blockIdx.x, wave = threadIdx.x >> 6;
+    const int p = batch_plane(t->filter_off, t->n, g);
+    filter_direct_body<FINAL, T2, T1, EXACT>(t->f[p], ((long long)(g - t->filter_off[p]) * 4 + wave) * 8,
+                                            threadIdx.x & 63, tbuf[wave]);
 }
 
 template <bool EXACT>
@@ -2676,6 +2705,26 @@ void launch_match_batch(int metric, bool shared_cols, const BatchTable *d, int w
     }
 }
 
+void launch_match_global_batch(int metric, const BatchTable *d, int wgs, hipStream_t s) {
+    switch (metric) {
+    case 0: hipLaunchKernelGGL((k_match_global_batch<0>), dim3(wgs), dim3(256), 0, s, d); break;
+    case 1: hipLaunchKernelGGL((k_match_global_batch<1>), dim3(wgs), dim3(256), 0, s, d); break;
+    case 2: hipLaunchKernelGGL((k_match_global_batch<2>), dim3(wgs), dim3(256), 0, s, d); break;
+    case 3: hipLaunchKernelGGL((k_match_global_batch<3>), dim3(wgs), dim3(256), 0, s, d); break;
+    default: hipLaunchKernelGGL((k_match_global_batch<4>), dim3(wgs), dim3(256), 0, s, d); break;
+    }
+}
+
+template <bool FINAL, bool EXACT>
+void launch_filter_direct_batch(int kind, const BatchTable *d, int wgs, hipStream_t s) {
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_filter_direct_batch<FINAL, 0, 0, EXACT>), dim3(wgs), dim3(256), 0, s, d); break;
+    case 1: hipLaunchKernelGGL((k_filter_direct_batch<FINAL, 1, 1, EXACT>), dim3(wgs), dim3(256), 0, s, d); break;
+    case 2: hipLaunchKernelGGL((k_filter_direct_batch<FINAL, 2, 2, EXACT>), dim3(wgs), dim3(256), 0, s, d); break;
+    default: hipLaunchKernelGGL((k_filter_direct_batch<FINAL, 3, 3, EXACT>), dim3(wgs), dim3(256), 0, s, d); break;
+    }
+}
+
 // Sizes the grid (as many workgroups as can be resident at once) when *grid is 0, then launches
 // on s unless size_only (s may be the null stream).
 template <int M, bool SC, int T>
@@ -2720,6 +2769,8 @@ struct vsz_bm3d_batch {
     int match_stride = 0;
     bool shared_cols = false;  // every plane takes the shared-column block_step-4 matcher
     bool sc_allowed = false;   // ... may take it (the autotuner then times both matchers)
+    bool global_match = false;  // windows beyond the LDS matchers' budget: the global-memory matcher
+    bool direct_filter = false; // a region beyond the filter tiles' LDS budget: direct-atomic filtering
     size_t lds_rows = 0, lds_sc = 0;
     int stride_rows = 0, stride_sc = 0;
     int schedule = VSZ_BATCH_STAGED;  // chosen schedule
@@ -2754,22 +2805,36 @@ struct vsz_bm3d_batch {
     }
 
     int passes() const { return bp.two_pass ? 2 : 1; }
+    // workgroups of one plane's filtering: tiles, or 32 groups each on the direct path
+    int filter_wgs_of(const BatchPlane &pl, int pass) const {
+        if (direct_filter) return (int)((pl.L.nblocks + 31) / 32);
+        return ((pl.L.nbx + pl.tile[pass].tbx - 1) / pl.tile[pass].tbx) * ((pl.L.nby + pl.tile[pass].tby - 1) / pl.tile[pass].tby);
+    }
     int filter_wgs(int pass) const {
         int n = 0;
-        for (const auto &pl : planes)
-            n += ((pl.L.nbx + pl.tile[pass].tbx - 1) / pl.tile[pass].tbx) * ((pl.L.nby + pl.tile[pass].tby - 1) / pl.tile[pass].tby);
+        for (const auto &pl : planes) n += filter_wgs_of(pl, pass);
         return n;
     }
     size_t filter_lds(int pass) const {
         size_t m = 0;
-        for (const auto &pl : planes) m = std::max(m, pl.tile[pass].lds_bytes);
+        if (!direct_filter)
+            for (const auto &pl : planes) m = std::max(m, pl.tile[pass].lds_bytes);
         return m;
     }
-    int match_tiles_x(const BatchPlane &pl) const { return shared_cols ? (pl.L.nbx + 14) / 15 : (pl.L.nbx + 15) / 16; }
-    int match_tiles_y(const BatchPlane &pl) const { return shared_cols ? (pl.L.nby + 7) / 8 : (pl.L.nby + 15) / 16; }
+    int match_tiles_x(const BatchPlane &pl) const {
+        return global_match ? (pl.L.nbx + 7) / 8 : shared_cols ? (pl.L.nbx + 14) / 15 : (pl.L.nbx + 15) / 16;
+    }
+    int match_tiles_y(const BatchPlane &pl) const {
+        return global_match ? (pl.L.nby + 7) / 8 : shared_cols ? (pl.L.nby + 7) / 8 : (pl.L.nby + 15) / 16;
+    }
+    // workgroups of one plane's matching (the global matcher: four 8 x 8-block tiles each)
+    int match_wgs_of(const BatchPlane &pl) const {
+        const int tiles = match_tiles_x(pl) * match_tiles_y(pl);
+        return global_match ? (tiles + 3) / 4 : tiles;
+    }
     int match_wgs() const {
         int n = 0;
-        for (const auto &pl : planes) n += match_tiles_x(pl) * match_tiles_y(pl);
+        for (const auto &pl : planes) n += match_wgs_of(pl);
         return n;
     }
     int finish_wgs() const {
@@ -2810,7 +2875,7 @@ struct vsz_bm3d_batch {
                 m.ts = pl.L.ts;
                 m.tiles_x = match_tiles_x(pl);
                 m.lds_stride = match_stride;
-                mo += m.tiles_x * match_tiles_y(pl);
+                mo += match_wgs_of(pl);
                 FilterArgs &f = t.f[i];
                 f = FilterArgs{};
                 f.matches = matches;
@@ -2837,8 +2902,8 @@ struct vsz_bm3d_batch {
                 f.tbx = pl.tile[pass].tbx;
                 f.tby = pl.tile[pass].tby;
                 f.guard_bits = pl.tile[pass].guard_bits;
-                t.filter_tiles_x[i] = (pl.L.nbx + f.tbx - 1) / f.tbx;
-                fo += t.filter_tiles_x[i] * ((pl.L.nby + f.tby - 1) / f.tby);
+                t.filter_tiles_x[i] = direct_filter ? 1 : (pl.L.nbx + f.tbx - 1) / f.tbx;
+                fo += filter_wgs_of(pl, pass);
                 FinishArgs &fa = t.fin[i];
                 fa = FinishArgs{};
                 fa.acc = acc;
@@ -2868,12 +2933,18 @@ struct vsz_bm3d_batch {
         for (int pass = 0; pass < passes(); ++pass) {
             const BatchTable *d = d_tab + pass;
             hipLaunchKernelGGL(k_zero, dim3(zgrid), dim3(256), 0, s, (unsigned long long *)(work + pass * acc_span), n64);
-            launch_match_batch(bp.metric, shared_cols, d, match_wgs(), match_lds, s);
+            if (global_match) launch_match_global_batch(bp.metric, d, match_wgs(), s);
+            else launch_match_batch(bp.metric, shared_cols, d, match_wgs(), match_lds, s);
             const int fw = filter_wgs(pass);
             const size_t fl = filter_lds(pass);
             const bool mf = bp.transform_precision == 1;
             const int kind = mf ? kDctMfma : bp.t2d;
-            if (pass == 0) {
+            if (direct_filter) {  // FP32 only (create rejects split-BF16 without tiles)
+                if (pass == 0) exact ? launch_filter_direct_batch<false, true>(kind, d, fw, s)
+                                     : launch_filter_direct_batch<false, false>(kind, d, fw, s);
+                else exact ? launch_filter_direct_batch<true, true>(kind, d, fw, s)
+                           : launch_filter_direct_batch<true, false>(kind, d, fw, s);
+            } else if (pass == 0) {
                 if (exact) launch_filter_batch_kind<false, true>(kind, d, fw, fl, s);
                 else launch_filter_batch_kind<false, false>(kind, d, fw, fl, s);
             } else {
@@ -2971,6 +3042,7 @@ struct vsz_bm3d_batch {
 // Times every candidate tile shape of the filter kernel on each distinct plane shape (both
 // passes, the plane's own noisy data as input, its own workspace) and keeps the fastest.
 int vsz_bm3d_batch::tune(const float *const *noisy, char *work, hipStream_t s) {
+    if (direct_filter) return VSZ_OK;  // no tile fits: nothing to tune
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return vsz_fail(VSZ_EDEVICE, "vsz_bm3d_batch_run: hipEventCreate failed");
@@ -3063,7 +3135,8 @@ int vsz_bm3d_batch::tune_schedule(const float *const *noisy, float *const *dst, 
         bool sc;
     };
     std::vector<Cand> cands;
-    for (int sc = 1; sc >= 0; --sc) {
+    if (global_match) cands.push_back({VSZ_BATCH_STAGED, false});  // the only form for such windows
+    for (int sc = 1; sc >= 0 && !global_match; --sc) {
         if (sc ? !sc_allowed : lds_rows == 0) continue;
         if (bp.schedule != VSZ_BATCH_STAGED && persist_ok) cands.push_back({VSZ_BATCH_PERSISTENT, sc != 0});
         if (bp.schedule != VSZ_BATCH_PERSISTENT) cands.push_back({VSZ_BATCH_STAGED, sc != 0});
@@ -3143,14 +3216,8 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
         b->stride_sc = ls;
         b->sc_allowed = b->lds_sc != 0;
     }
-    if (!b->lds_rows && !b->sc_allowed)
-        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: block_step/bm_range too large for the LDS-staged matcher");
+    b->global_match = !b->lds_rows && !b->sc_allowed;  // windows beyond the LDS matchers' budget
     b->use_matcher(b->sc_allowed);
-    // the persistent form: exact aggregation, ssd or sad
-    b->persist_ok = b->exact && (bp->metric == 0 || bp->metric == 1) && bp->t2d == 0;
-    if (bp->schedule == VSZ_BATCH_PERSISTENT && !b->persist_ok)
-        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: the persistent schedule needs extractor_exp > 0, the ssd or sad "
-                                    "metric and the dct transform");
     const float fac[2] = {bits_float(0x3f021bb6u), bits_float(0x3e40c0c1u)};  // src/bm3d.zig:1013-1024
     size_t acc = 0, rest = 0;
     for (int i = 0; i < n; ++i) {
@@ -3177,12 +3244,23 @@ int vsz_bm3d_batch_create(const vsz_bm3d_batch_params *bp, const vsz_bm3d_plane 
             if (!kparams_ok(&k)) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: invalid geometry", i);
             pl.L = layout_of(&k);
             pl.tile[pass] = plan_tiles(&k, pl.L, 0);
-            if (!pl.tile[pass].ok) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: no filter tile fits", i);
+            if (!pl.tile[pass].ok) {  // a search region beyond the tiles' LDS budget: direct atomics
+                if (bp->transform_precision == 1)
+                    return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: plane %d: no filter tile fits (split-BF16 "
+                                                "batches need LDS tiles)", i);
+                b->direct_filter = true;
+            }
         }
         pl.acc_off[0] = acc;
         acc += (pl.L.acc_bytes + 255) & ~size_t(255);
         b->planes.push_back(pl);
     }
+    // the persistent form: exact aggregation, ssd or sad, dct, the LDS matchers and filter tiles
+    b->persist_ok = b->exact && (bp->metric == 0 || bp->metric == 1) && bp->t2d == 0 && !b->global_match &&
+                    !b->direct_filter;
+    if (bp->schedule == VSZ_BATCH_PERSISTENT && !b->persist_ok)
+        return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_create: the persistent schedule needs extractor_exp > 0, the ssd or sad "
+                                    "metric, the dct transform and windows within the LDS budgets");
     // [pass 0 accumulators][pass 1 accumulators] then per plane [matches per pass][basic], every
     // region 256-byte aligned (no cache line holds data of two regions)
     b->acc_span = acc;
@@ -3314,7 +3392,7 @@ int vsz_bm3d_batch_check(vsz_bm3d_batch *b, void *stream) {
 int vsz_bm3d_batch_schedule(const vsz_bm3d_batch *b, int *schedule, int *shared_cols) {
     if (!b || !schedule || !shared_cols) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_batch_schedule: invalid arguments");
     *schedule = b->schedule;
-    *shared_cols = b->shared_cols ? 1 : 0;
+    *shared_cols = b->global_match ? 2 : b->shared_cols ? 1 : 0;
     return VSZ_OK;
 }
 

@@ -190,10 +190,11 @@ class Batch:
         _check(_lib.load().vsz_bm3d_batch_check(self._h, _stream_handle(stream)))
 
     def schedule(self):
-        """(schedule name, shared-column matcher) the batch runs with; final after its first run."""
-        sch, sc = C.c_int(), C.c_int()
-        _check(_lib.load().vsz_bm3d_batch_schedule(self._h, C.byref(sch), C.byref(sc)))
-        return {0: "staged", 1: "persistent"}[sch.value], bool(sc.value)
+        """(schedule, matcher) the batch runs with, e.g. ("staged", "shared-column"); final after its
+        first run. Matchers: "rows", "shared-column", "global" (windows beyond the LDS budget)."""
+        sch, m = C.c_int(), C.c_int()
+        _check(_lib.load().vsz_bm3d_batch_schedule(self._h, C.byref(sch), C.byref(m)))
+        return {0: "staged", 1: "persistent"}[sch.value], {0: "rows", 1: "shared-column", 2: "global"}[m.value]
 
     def tile(self, plane: int, pass_: int):
         tx, ty = C.c_int(), C.c_int()
