@@ -1547,7 +1547,12 @@ __device__ __forceinline__ float pair_shrink(vsz_f4 (&z)[4], const vsz_f4 (&r)[4
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float r2 = r[s][e] * r[s][e];
-                float coeff = __fdiv_rn(r2, r2 + s2);
+                // reciprocal and one Newton step (within an ulp of the division; the split form is
+                // held to error bounds, not bits, and the reference itself divides with fast math)
+                const float den = r2 + s2;
+                float rc = __builtin_amdgcn_rcpf(den);
+                rc = fmaf(fmaf(-den, rc, 1.0f), rc, rc);
+                float coeff = r2 * rc;
                 if (s == 0 && e == 0 && dc_lane) coeff = 1.0f;
                 z[s][e] = z[s][e] * coeff;
                 k = fmaf_(coeff, coeff, k);
@@ -1781,7 +1786,7 @@ __global__ __launch_bounds__(64) void k_filter_mfma(FilterArgs a) {
 }
 
 template <bool FINAL, int T2, int T1, bool EXACT>
-__global__ __launch_bounds__(256, 2) void k_filter_tile(FilterArgs a) {
+__global__ __launch_bounds__(256, T2 == kDctMfma ? 4 : 2) void k_filter_tile(FilterArgs a) {
     extern __shared__ float smem[];
     filter_tile_body<FINAL, T2, T1, EXACT>(a, blockIdx.x, blockIdx.y, smem);
 }
@@ -1918,7 +1923,7 @@ __global__ __launch_bounds__(256) void k_match_lds4_batch(const BatchTable *__re
 }
 
 template <bool FINAL, int T2, int T1, bool EXACT>
-__global__ __launch_bounds__(256, 2) void k_filter_tile_batch(const BatchTable *__restrict__ t) {
+__global__ __launch_bounds__(256, T2 == kDctMfma ? 4 : 2) void k_filter_tile_batch(const BatchTable *__restrict__ t) {
     extern __shared__ float smem[];
     const int g = xcd_tile(blockIdx.x, gridDim.x);
     const int p = batch_plane(t->filter_off, t->n, g);
