@@ -2295,7 +2295,10 @@ size_t transpose_bytes(const vsz_bm3d_kparams *p) {
 // MFMA dependency chains) runs five: a measured sweep of 28, 32, 40 and 48 KB on the 1080p bench
 // frame put 32 KB first (1.56 ms per two-pass step, against 1.62, 1.59 and 1.74).
 size_t region_budget(const vsz_bm3d_kparams *p) {
-    return p->transform_precision == 1 ? size_t(32) * 1024 : kRegionBudgetBytes;
+    // split-BF16 tiles: as many workgroups per CU as their registers allow (five for the hard
+    // pass, four for Wiener) share the CU's 160 KB
+    if (p->transform_precision == 1) return size_t(p->final_pass ? 40 : 32) * 1024;
+    return kRegionBudgetBytes;
 }
 
 struct TilePlan {
