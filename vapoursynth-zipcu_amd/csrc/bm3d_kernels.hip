@@ -1786,7 +1786,10 @@ __global__ __launch_bounds__(64) void k_filter_mfma(FilterArgs a) {
 }
 
 template <bool FINAL, int T2, int T1, bool EXACT>
-__global__ __launch_bounds__(256, T2 == kDctMfma ? 4 : 2) void k_filter_tile(FilterArgs a) {
+// Waves per SIMD the registers must allow: four for split-BF16 tiles, three for the FP32 hard
+// pass (168 VGPRs, so three workgroups share a CU), two for the FP32 Wiener pass (two groups of
+// 64 values per lane).
+__global__ __launch_bounds__(256, T2 == kDctMfma ? 4 : (FINAL ? 2 : 3)) void k_filter_tile(FilterArgs a) {
     extern __shared__ float smem[];
     filter_tile_body<FINAL, T2, T1, EXACT>(a, blockIdx.x, blockIdx.y, smem);
 }
@@ -2291,7 +2294,8 @@ size_t transpose_bytes(const vsz_bm3d_kparams *p) {
     return p->transform_precision == 1 ? kPairTransposeBytes : kTransposeBytes;
 }
 
-// LDS per workgroup: two FP32 workgroups per CU; the split-BF16 kernel (fewer registers, longer
+// LDS per workgroup: two FP32 workgroups per CU (the single-frame hard pass may take three, see
+// plan_tiles_frame); the split-BF16 kernel (fewer registers, longer
 // MFMA dependency chains) runs five: a measured sweep of 28, 32, 40 and 48 KB on the 1080p bench
 // frame put 32 KB first (1.56 ms per two-pass step, against 1.62, 1.59 and 1.74).
 size_t region_budget(const vsz_bm3d_kparams *p) {
@@ -2345,6 +2349,50 @@ TilePlan plan_tiles(const vsz_bm3d_kparams *p, const Layout &L, int min_wgs = 51
             }
         }
     if (best.ok) best.guard_bits = guard_bits_for(best.tbx, best.tby);
+    return best;
+}
+
+// Tile plan of the single-frame FP32 hard pass, whose kernel can run three workgroups per CU
+// (768 on 256 CUs) when a tile's region fits a third of the CU's 160 KB. Used when the plane
+// splits into one near-full round of such workgroups (691-768) with at least 128 groups per tile;
+// otherwise plan_tiles decides (small planes, wide windows, large planes). Among those shapes:
+// the fewest 32-group iterations per workgroup (a wave filters 8 groups at a time), then the
+// most workgroups (the fullest round), then the fewest flushed cells. On the 1080p bench frame
+// (block_step 4, bm_range 16) this picks 10x17 tiles: basic pass 0.659 ms against 0.678 with
+// two 16x16 tiles per CU (three interleaved runs each; 16x11 0.656, 12x15 0.665, 12x16 0.666).
+TilePlan plan_tiles_frame(const vsz_bm3d_kparams *p, const Layout &L) {
+    if (p->transform_precision != 0 || p->final_pass || p->radius != 0) return plan_tiles(p, L);
+    constexpr long long kSlots = 3 * 256, kMinWgs = 691;
+    constexpr size_t kBudget = 53 * 1024;
+    TilePlan best;
+    int best_iters = 0;
+    long long best_wgs = 0, best_cells = 0;
+    for (int tbx = 1; tbx <= 32; ++tbx)
+        for (int tby = 1; tby <= 32; ++tby) {
+            const int groups = tbx * tby;
+            if (groups < 128 || groups > 512 || tbx > L.nbx || tby > L.nby) continue;
+            const long long rw = std::min<long long>((long long)(tbx - 1) * p->block_step + 8 + 2LL * p->bm_range, p->width);
+            const long long rh = std::min<long long>((long long)(tby - 1) * p->block_step + 8 + 2LL * p->bm_range, p->height);
+            const size_t bytes = transpose_bytes(p) + (size_t)(rw * rh) * 4;
+            if (bytes > kBudget) continue;
+            const long long wgs = (long long)((L.nbx + tbx - 1) / tbx) * ((L.nby + tby - 1) / tby);
+            if (wgs < kMinWgs || wgs > kSlots) continue;
+            const int iters = (groups + 31) / 32;
+            const long long cells = wgs * rw * rh;
+            const bool better = !best.ok || iters < best_iters ||
+                                (iters == best_iters && (wgs > best_wgs || (wgs == best_wgs && cells < best_cells)));
+            if (better) {
+                best.ok = true;
+                best.tbx = tbx;
+                best.tby = tby;
+                best.lds_bytes = bytes;
+                best_iters = iters;
+                best_wgs = wgs;
+                best_cells = cells;
+            }
+        }
+    if (!best.ok) return plan_tiles(p, L);
+    best.guard_bits = guard_bits_for(best.tbx, best.tby);
     return best;
 }
 
@@ -2587,7 +2635,7 @@ int run_pass(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_cli
     f.bm_range = p->bm_range;
     f.nbx = L.nbx;
     f.nby = L.nby;
-    const TilePlan tp = plan_tiles(p, L);
+    const TilePlan tp = plan_tiles_frame(p, L);
     f.tbx = tp.tbx;
     f.tby = tp.tby;
     f.guard_bits = tp.guard_bits;
