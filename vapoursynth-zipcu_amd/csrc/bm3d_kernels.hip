@@ -670,10 +670,22 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
     for (int k = 0; k < 8; ++k) out[k] = make_int4(pos_x(L.pos[k]), pos_y(L.pos[k]), 0, __float_as_int(L.er[k]));
 }
 
+// Clears this workgroup's share of the accumulator that the pass's filtering adds into (the
+// filter runs after the matcher on the same stream). Issued after the search: the stores drain
+// while the other waves of the CU keep the VALU busy, instead of taking a bandwidth-bound kernel
+// of their own ahead of the matcher.
+__device__ __forceinline__ void zero_share(const MatchArgs &a) {
+    if (!a.zero) return;
+    const long long per = (a.zero_n64 + gridDim.x - 1) / gridDim.x;
+    const long long b = (long long)blockIdx.x * per, e = b + per < a.zero_n64 ? b + per : a.zero_n64;
+    for (long long i = b + threadIdx.x; i < e; i += blockDim.x) a.zero[i] = 0ull;
+}
+
 template <int M, int WR, int SPLIT>
 __global__ __launch_bounds__(128 * WR) void k_match_lds(MatchArgs a) {
     extern __shared__ float region[];
     match_lds_body<M, WR, SPLIT>(a, xcd_tile(blockIdx.x, gridDim.x), region);
+    zero_share(a);
 }
 
 // ---- shared-column matching for block_step 4 (ssd, sad) ---------------------------------------
@@ -828,6 +840,7 @@ template <int M>
 __global__ __launch_bounds__(256) void k_match_lds4(MatchArgs a) {
     extern __shared__ float region[];
     match_lds4_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), region);
+    zero_share(a);
 }
 
 // Spatial search on raw buffer loads, for windows whose staged region exceeds the LDS budget: one
@@ -877,6 +890,7 @@ __device__ __forceinline__ void match_global_body(const MatchArgs &a, int tile, 
 template <int M>
 __global__ __launch_bounds__(64) void k_match(MatchArgs a) {
     match_global_body<M>(a, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x);
+    zero_share(a);
 }
 
 // Temporal matching after the centre frame's spatial search: reads the centre lists, runs the
@@ -2262,6 +2276,7 @@ template <int M> void launch_match_m(const MatchArgs &a, hipStream_t s) {
     c.no_self = 1;
     launch_spatial<M>(c, plan_match(c, M), s);
     MatchArgs t = a;
+    t.zero = nullptr;  // the centre frame's spatial search cleared it
     t.tiles_x = (a.nbx + 7) / 8;
     if (a.ps_num == 2) {  // one lane per window
         const dim3 grid((unsigned)(t.tiles_x * ((a.nby + 1) / 2)), 1, 1);
@@ -2463,6 +2478,32 @@ int check_launch(const char *what) {
     return VSZ_OK;
 }
 
+// The search of vsz_bm3d_match; with zero non-null its spatial kernel also clears
+// zero[0, zero_n64) (the accumulator of the pass that follows).
+int match_and_clear(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_matches, unsigned long long *zero,
+                    long long zero_n64, hipStream_t s) {
+    const Layout L = layout_of(p);
+    MatchArgs m{};
+    m.img = d_img;
+    m.matches = reinterpret_cast<int4 *>(d_matches);
+    m.width = p->width;
+    m.height = p->height;
+    m.stride = p->stride;
+    m.nbx = L.nbx;
+    m.nby = L.nby;
+    m.block_step = p->block_step;
+    m.bm_range = p->bm_range;
+    m.radius = p->radius;
+    m.ps_num = p->ps_num;
+    m.ps_range = p->ps_range;
+    m.ts = L.ts;
+    m.tiles_x = (L.nbx + 7) / 8;
+    m.zero = zero;
+    m.zero_n64 = zero_n64;
+    launch_match(m, p->metric, s);
+    return check_launch("vsz_bm3d_match");
+}
+
 }  // namespace
 
 extern "C" {
@@ -2483,24 +2524,7 @@ size_t vsz_bm3d_workspace_size(const vsz_bm3d_kparams *p) {
 
 int vsz_bm3d_match(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_matches, void *stream) {
     if (!kparams_ok(p) || !d_img || !d_matches) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_match: invalid arguments");
-    const Layout L = layout_of(p);
-    MatchArgs m{};
-    m.img = d_img;
-    m.matches = reinterpret_cast<int4 *>(d_matches);
-    m.width = p->width;
-    m.height = p->height;
-    m.stride = p->stride;
-    m.nbx = L.nbx;
-    m.nby = L.nby;
-    m.block_step = p->block_step;
-    m.bm_range = p->bm_range;
-    m.radius = p->radius;
-    m.ps_num = p->ps_num;
-    m.ps_range = p->ps_range;
-    m.ts = L.ts;
-    m.tiles_x = (L.nbx + 7) / 8;
-    launch_match(m, p->metric, (hipStream_t)stream);
-    return check_launch("vsz_bm3d_match");
+    return match_and_clear(p, d_img, d_matches, nullptr, 0, (hipStream_t)stream);
 }
 
 const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p) {
@@ -2599,13 +2623,10 @@ int run_pass(const vsz_bm3d_kparams *p, const float *d_clip0, const float *d_cli
     int4 *matches = reinterpret_cast<int4 *>(d_work);
     void *acc = static_cast<char *>(d_work) + L.match_bytes;
 
-    // zero the accumulators with a kernel (graph-capture safe, no memset node)
-    {
-        const long long n64 = (long long)(L.acc_bytes / 8);
-        const unsigned grid = (unsigned)std::min<long long>((n64 + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, s, (unsigned long long *)acc, n64);
-    }
-    int rc = vsz_bm3d_match(p, d_clip0, reinterpret_cast<int32_t *>(matches), stream);
+    // the spatial search also clears the accumulators (kernel stores: graph-capture safe, no
+    // memset node); the filter below runs after it on the same stream
+    int rc = match_and_clear(p, d_clip0, reinterpret_cast<int32_t *>(matches), (unsigned long long *)acc,
+                             (long long)(L.acc_bytes / 8), s);
     if (rc) return rc;
 
     FilterArgs f{};

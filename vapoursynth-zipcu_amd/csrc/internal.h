@@ -32,6 +32,8 @@ struct MatchArgs {
     int tiles_x;     // block tiles per row
     int lds_stride;  // LDS-staged path: floats per staged row
     int no_self;     // 1: leave out the self-inclusion (a temporal stage follows)
+    unsigned long long *zero;  // non-null: the spatial search also clears zero[0, zero_n64)
+    long long zero_n64;
 };
 
 struct FilterArgs {
