@@ -674,12 +674,14 @@ __device__ __forceinline__ void match_lds_body(const MatchArgs &a, int tile, flo
 // filter runs after the matcher on the same stream). Issued after the search: the stores drain
 // while the other waves of the CU keep the VALU busy, instead of taking a bandwidth-bound kernel
 // of their own ahead of the matcher.
-__device__ __forceinline__ void zero_share(const MatchArgs &a) {
-    if (!a.zero) return;
-    const long long per = (a.zero_n64 + gridDim.x - 1) / gridDim.x;
-    const long long b = (long long)blockIdx.x * per, e = b + per < a.zero_n64 ? b + per : a.zero_n64;
-    for (long long i = b + threadIdx.x; i < e; i += blockDim.x) a.zero[i] = 0ull;
+__device__ __forceinline__ void zero_range(unsigned long long *z, long long n64) {
+    if (!z) return;
+    const long long per = (n64 + gridDim.x - 1) / gridDim.x;
+    const long long b = (long long)blockIdx.x * per, e = b + per < n64 ? b + per : n64;
+    for (long long i = b + threadIdx.x; i < e; i += blockDim.x) z[i] = 0ull;
 }
+
+__device__ __forceinline__ void zero_share(const MatchArgs &a) { zero_range(a.zero, a.zero_n64); }
 
 template <int M, int WR, int SPLIT>
 __global__ __launch_bounds__(128 * WR) void k_match_lds(MatchArgs a) {
@@ -1872,10 +1874,6 @@ __global__ __launch_bounds__(256) void k_finish(FinishArgs a) {
     finish_body<EXACT>(a, blockIdx.x, blockIdx.y, blockIdx.z, sD, sH);
 }
 
-__global__ __launch_bounds__(256) void k_zero(unsigned long long *p, long long n64) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n64; i += (long long)gridDim.x * 256) p[i] = 0ull;
-}
-
 // Temporal aggregation (src/bm3d_vagg.cu:18-48): ascending FP32 sums, one IEEE division.
 __global__ __launch_bounds__(256) void k_vaggregate(VaggArgs a) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -1921,22 +1919,27 @@ __device__ __forceinline__ int batch_plane(const int *off, int n, int g) {
     return lo;
 }
 
+// The batch matchers also clear the pass's accumulator span, zero[0, zero_n64) (see zero_range).
 template <int M>
-__global__ __launch_bounds__(256) void k_match_lds_batch(const BatchTable *__restrict__ t) {
+__global__ __launch_bounds__(256) void k_match_lds_batch(const BatchTable *__restrict__ t, unsigned long long *zero,
+                                                         long long zero_n64) {
     extern __shared__ float region[];
     const int g = xcd_tile(blockIdx.x, gridDim.x);
     const int p = batch_plane(t->match_off, t->n, g);
     const MatchArgs &a = t->m[p];
     match_lds_body<M>(a, g - t->match_off[p], region);
+    zero_range(zero, zero_n64);
 }
 
 template <int M>
-__global__ __launch_bounds__(256) void k_match_lds4_batch(const BatchTable *__restrict__ t) {
+__global__ __launch_bounds__(256) void k_match_lds4_batch(const BatchTable *__restrict__ t, unsigned long long *zero,
+                                                          long long zero_n64) {
     extern __shared__ float region[];
     const int g = xcd_tile(blockIdx.x, gridDim.x);
     const int p = batch_plane(t->match_off, t->n, g);
     const MatchArgs &a = t->m[p];
     match_lds4_body<M>(a, g - t->match_off[p], region);
+    zero_range(zero, zero_n64);
 }
 
 template <bool FINAL, int T2, int T1, bool EXACT>
@@ -1952,13 +1955,14 @@ __global__ __launch_bounds__(256, T2 == kDctMfma ? 4 : 2) void k_filter_tile_bat
 // Windows beyond the LDS budgets: the global-memory matcher (each wave one 8 x 8-block tile) and
 // the direct-atomic filter (each wave 8 groups), over a batch.
 template <int M>
-__global__ __launch_bounds__(256) void k_match_global_batch(const BatchTable *__restrict__ t) {
+__global__ __launch_bounds__(256) void k_match_global_batch(const BatchTable *__restrict__ t,
+                                                            unsigned long long *zero, long long zero_n64) {
     const int g = blockIdx.x;
     const int p = batch_plane(t->match_off, t->n, g);
     const MatchArgs &a = t->m[p];
     const int tile = (g - t->match_off[p]) * 4 + (threadIdx.x >> 6);
-    if (tile >= a.tiles_x * ((a.nby + 7) / 8)) return;
-    match_global_body<M>(a, tile, threadIdx.x & 63);
+    if (tile < a.tiles_x * ((a.nby + 7) / 8)) match_global_body<M>(a, tile, threadIdx.x & 63);
+    zero_range(zero, zero_n64);
 }
 
 template <bool FINAL, int T2, int T1, bool EXACT>
@@ -2767,28 +2771,30 @@ void launch_filter_batch_kind(int kind, const BatchTable *d, int wgs, size_t lds
     }
 }
 
-void launch_match_batch(int metric, bool shared_cols, const BatchTable *d, int wgs, size_t lds, hipStream_t s) {
+void launch_match_batch(int metric, bool shared_cols, const BatchTable *d, int wgs, size_t lds, unsigned long long *zero,
+                        long long n64, hipStream_t s) {
     if (shared_cols) {
-        if (metric == 0) hipLaunchKernelGGL((k_match_lds4_batch<0>), dim3(wgs), dim3(256), lds, s, d);
-        else hipLaunchKernelGGL((k_match_lds4_batch<1>), dim3(wgs), dim3(256), lds, s, d);
+        if (metric == 0) hipLaunchKernelGGL((k_match_lds4_batch<0>), dim3(wgs), dim3(256), lds, s, d, zero, n64);
+        else hipLaunchKernelGGL((k_match_lds4_batch<1>), dim3(wgs), dim3(256), lds, s, d, zero, n64);
         return;
     }
     switch (metric) {
-    case 0: hipLaunchKernelGGL((k_match_lds_batch<0>), dim3(wgs), dim3(256), lds, s, d); break;
-    case 1: hipLaunchKernelGGL((k_match_lds_batch<1>), dim3(wgs), dim3(256), lds, s, d); break;
-    case 2: hipLaunchKernelGGL((k_match_lds_batch<2>), dim3(wgs), dim3(256), lds, s, d); break;
-    case 3: hipLaunchKernelGGL((k_match_lds_batch<3>), dim3(wgs), dim3(256), lds, s, d); break;
-    default: hipLaunchKernelGGL((k_match_lds_batch<4>), dim3(wgs), dim3(256), lds, s, d); break;
+    case 0: hipLaunchKernelGGL((k_match_lds_batch<0>), dim3(wgs), dim3(256), lds, s, d, zero, n64); break;
+    case 1: hipLaunchKernelGGL((k_match_lds_batch<1>), dim3(wgs), dim3(256), lds, s, d, zero, n64); break;
+    case 2: hipLaunchKernelGGL((k_match_lds_batch<2>), dim3(wgs), dim3(256), lds, s, d, zero, n64); break;
+    case 3: hipLaunchKernelGGL((k_match_lds_batch<3>), dim3(wgs), dim3(256), lds, s, d, zero, n64); break;
+    default: hipLaunchKernelGGL((k_match_lds_batch<4>), dim3(wgs), dim3(256), lds, s, d, zero, n64); break;
     }
 }
 
-void launch_match_global_batch(int metric, const BatchTable *d, int wgs, hipStream_t s) {
+void launch_match_global_batch(int metric, const BatchTable *d, int wgs, unsigned long long *zero, long long n64,
+                               hipStream_t s) {
     switch (metric) {
-    case 0: hipLaunchKernelGGL((k_match_global_batch<0>), dim3(wgs), dim3(256), 0, s, d); break;
-    case 1: hipLaunchKernelGGL((k_match_global_batch<1>), dim3(wgs), dim3(256), 0, s, d); break;
-    case 2: hipLaunchKernelGGL((k_match_global_batch<2>), dim3(wgs), dim3(256), 0, s, d); break;
-    case 3: hipLaunchKernelGGL((k_match_global_batch<3>), dim3(wgs), dim3(256), 0, s, d); break;
-    default: hipLaunchKernelGGL((k_match_global_batch<4>), dim3(wgs), dim3(256), 0, s, d); break;
+    case 0: hipLaunchKernelGGL((k_match_global_batch<0>), dim3(wgs), dim3(256), 0, s, d, zero, n64); break;
+    case 1: hipLaunchKernelGGL((k_match_global_batch<1>), dim3(wgs), dim3(256), 0, s, d, zero, n64); break;
+    case 2: hipLaunchKernelGGL((k_match_global_batch<2>), dim3(wgs), dim3(256), 0, s, d, zero, n64); break;
+    case 3: hipLaunchKernelGGL((k_match_global_batch<3>), dim3(wgs), dim3(256), 0, s, d, zero, n64); break;
+    default: hipLaunchKernelGGL((k_match_global_batch<4>), dim3(wgs), dim3(256), 0, s, d, zero, n64); break;
     }
 }
 
@@ -3003,15 +3009,15 @@ struct vsz_bm3d_batch {
         }
     }
 
-    // All launches of one run, on stream s (captured into the graph).
+    // All launches of one run, on stream s (captured into the graph); each pass's matcher also
+    // clears that pass's accumulator span.
     void enqueue(char *work, hipStream_t s) {
         const long long n64 = (long long)(acc_span / 8);
-        const unsigned zgrid = (unsigned)std::min<long long>((n64 + 255) / 256, 8192);
         for (int pass = 0; pass < passes(); ++pass) {
             const BatchTable *d = d_tab + pass;
-            hipLaunchKernelGGL(k_zero, dim3(zgrid), dim3(256), 0, s, (unsigned long long *)(work + pass * acc_span), n64);
-            if (global_match) launch_match_global_batch(bp.metric, d, match_wgs(), s);
-            else launch_match_batch(bp.metric, shared_cols, d, match_wgs(), match_lds, s);
+            unsigned long long *acc = (unsigned long long *)(work + pass * acc_span);
+            if (global_match) launch_match_global_batch(bp.metric, d, match_wgs(), acc, n64, s);
+            else launch_match_batch(bp.metric, shared_cols, d, match_wgs(), match_lds, acc, n64, s);
             const int fw = filter_wgs(pass);
             const size_t fl = filter_lds(pass);
             const bool mf = bp.transform_precision == 1;
