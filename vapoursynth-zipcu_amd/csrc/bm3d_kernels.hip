@@ -1801,10 +1801,10 @@ __global__ __launch_bounds__(64) void k_filter_mfma(FilterArgs a) {
     }
 }
 
-template <bool FINAL, int T2, int T1, bool EXACT>
 // Waves per SIMD the registers must allow: four for split-BF16 tiles, three for the FP32 hard
 // pass (168 VGPRs, so three workgroups share a CU), two for the FP32 Wiener pass (two groups of
 // 64 values per lane).
+template <bool FINAL, int T2, int T1, bool EXACT>
 __global__ __launch_bounds__(256, T2 == kDctMfma ? 4 : (FINAL ? 2 : 3)) void k_filter_tile(FilterArgs a) {
     extern __shared__ float smem[];
     filter_tile_body<FINAL, T2, T1, EXACT>(a, blockIdx.x, blockIdx.y, smem);
@@ -2314,9 +2314,9 @@ size_t transpose_bytes(const vsz_bm3d_kparams *p) {
 }
 
 // LDS per workgroup: two FP32 workgroups per CU (the single-frame hard pass may take three, see
-// plan_tiles_frame); the split-BF16 kernel (fewer registers, longer
-// MFMA dependency chains) runs five: a measured sweep of 28, 32, 40 and 48 KB on the 1080p bench
-// frame put 32 KB first (1.56 ms per two-pass step, against 1.62, 1.59 and 1.74).
+// plan_tiles_frame); the split-BF16 kernel (fewer registers, longer MFMA dependency chains) runs
+// five: a measured sweep of 28, 32, 40 and 48 KB on the 1080p bench frame put 32 KB first (1.56 ms
+// per two-pass step, against 1.62, 1.59 and 1.74).
 size_t region_budget(const vsz_bm3d_kparams *p) {
     // split-BF16 tiles: as many workgroups per CU as their registers allow (five for the hard
     // pass, four for Wiener) share the CU's 160 KB
