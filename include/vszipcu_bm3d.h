@@ -81,6 +81,15 @@ int vsz_bm3d_match(const vsz_bm3d_kparams *p, const float *d_img, int32_t *d_mat
  * search). "" for invalid parameters. The returned string is static. */
 const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p);
 
+/* Diagnostic, no reference counterpart: the filter vsz_bm3d_run would launch for these
+ * parameters. Radius 0 takes the LDS-tiled filter: *tbx x *tby reference blocks per 256-thread
+ * workgroup, *per_cu the workgroups that registers and LDS let share one CU (at most 2 for the
+ * FP32 Wiener pass, 3 for the FP32 hard pass, 4 / 5 for split-BF16 Wiener / hard). The hard pass
+ * of a plane that fills one round of 768 such workgroups gets a tile shape planned for three per
+ * CU. 0 x 0 tiles (and *per_cu 0) mean the direct-atomic filter: radius > 0, or a search region
+ * beyond the LDS budget. Returns VSZ_OK or VSZ_EINVAL. */
+int vsz_bm3d_filter_plan(const vsz_bm3d_kparams *p, int *tbx, int *tby, int *per_cu);
+
 /* One full pass: matching, collaborative filtering and aggregation.
  *   d_clip0: [P][TW][H][stride]  basic estimate (final pass) or noisy clip (basic pass)
  *   d_clip1: [P][TW][H][stride]  noisy clip (final pass only, else ignored)

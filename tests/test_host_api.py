@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for h in ("vszipcu_bm3d.h", "vszipcu_cnn.h"):
         header = open(os.path.join(REPO, "include", h)).read()
         names |= set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(vsz_\w+)\s*\(", header, re.M))
-    assert len(names) == 33
+    assert len(names) == 34
     for n in sorted(names):
         assert hasattr(L, n), n
     assert set(_lib.PROTOTYPES) == names
@@ -170,3 +170,32 @@ def test_match_plan_selects_the_parity_tested_kernels(kw, plan):
     from vszipcu import device
     w, h = kw.pop("width"), kw.pop("height")
     assert device.match_plan(device.kparams(w, h, **kw)) == plan
+
+
+@pytest.mark.parametrize("kw,plan", [
+    # the 1080p bench frame's hard pass: one round of 768 workgroups, three per CU
+    (dict(width=1920, height=1080, block_step=4, bm_range=16, chroma=True), (10, 17, 3)),
+    # its Wiener pass keeps two 16 x 16-block tiles per CU (245 VGPRs)
+    (dict(width=1920, height=1080, block_step=4, bm_range=16, chroma=True, final=True), (16, 16, 2)),
+    # small planes cannot fill a round of three: the planner for two per CU (small tiles, which
+    # then fit three per CU anyway)
+    (dict(width=640, height=360, block_step=8, bm_range=9), (2, 4, 3)),
+    (dict(width=640, height=360, block_step=8, bm_range=9, final=True), (2, 4, 2)),
+    # split-BF16 tiles
+    (dict(width=1920, height=1080, block_step=4, bm_range=16, chroma=True, precision="split-bf16"), None),
+    # radius > 0 and windows beyond the LDS budget filter with direct atomics
+    (dict(width=1920, height=1080, block_step=8, bm_range=9, radius=2), (0, 0, 0)),
+    (dict(width=640, height=480, block_step=8, bm_range=120), (0, 0, 0)),
+    # a window beyond the matcher's LDS budget still filters in single-block tiles
+    (dict(width=200, height=150, block_step=8, bm_range=48), (1, 1, 3)),
+])
+def test_filter_plan(kw, plan):
+    """Tile plans of the LDS-tiled filter (the three-per-CU hard-pass plan is parity-tested on the
+    GPU by the full-size 1080p test)."""
+    from vszipcu import device
+    w, h = kw.pop("width"), kw.pop("height")
+    got = device.filter_plan(device.kparams(w, h, **kw))
+    if plan is None:  # split-BF16 hard pass: 32 KB regions, five workgroups per CU
+        assert got[0] > 0 and got[1] > 0 and got[2] == 5, got
+    else:
+        assert got == plan

@@ -2552,6 +2552,21 @@ const char *vsz_bm3d_match_plan(const vsz_bm3d_kparams *p) {
     return p->radius > 0 ? temporal_names[pl.kernel] : names[pl.kernel];
 }
 
+int vsz_bm3d_filter_plan(const vsz_bm3d_kparams *p, int *tbx, int *tby, int *per_cu) {
+    if (!kparams_ok(p) || !tbx || !tby || !per_cu) return vsz_fail(VSZ_EINVAL, "vsz_bm3d_filter_plan: invalid arguments");
+    const Layout L = layout_of(p);
+    const TilePlan tp = plan_tiles_frame(p, L);
+    *tbx = tp.ok ? tp.tbx : 0;
+    *tby = tp.ok ? tp.tby : 0;
+    int n = 0;
+    if (tp.ok) {
+        const int by_regs = p->transform_precision == 1 ? (p->final_pass ? 4 : 5) : (p->final_pass ? 2 : 3);
+        n = std::min<int>(by_regs, (int)((160 * 1024) / tp.lds_bytes));
+    }
+    *per_cu = n;
+    return VSZ_OK;
+}
+
 }  // extern "C"
 
 namespace {

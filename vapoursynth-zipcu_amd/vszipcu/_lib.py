@@ -99,6 +99,7 @@ PROTOTYPES = {
     "vsz_bm3d_workspace_size": (C.c_size_t, [C.POINTER(KParams)]),
     "vsz_bm3d_match": (C.c_int, [C.POINTER(KParams), C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsz_bm3d_match_plan": (C.c_char_p, [C.POINTER(KParams)]),
+    "vsz_bm3d_filter_plan": (C.c_int, [C.POINTER(KParams), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vsz_bm3d_run": (C.c_int, [C.POINTER(KParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                C.c_void_p]),
     "vsz_bm3d_run_noalloc": (C.c_int, [C.POINTER(KParams), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,

@@ -73,6 +73,14 @@ def match_plan(p: _lib.KParams) -> str:
     return name
 
 
+def filter_plan(p: _lib.KParams):
+    """(tbx, tby, workgroups per CU) of the LDS-tiled filter vsz_bm3d_run launches for these
+    parameters, or (0, 0, 0) for the direct-atomic filter (no GPU needed)."""
+    tx, ty, n = C.c_int(), C.c_int(), C.c_int()
+    _check(_lib.load().vsz_bm3d_filter_plan(C.byref(p), C.byref(tx), C.byref(ty), C.byref(n)))
+    return tx.value, ty.value, n.value
+
+
 def workspace_size(p: _lib.KParams) -> int:
     n = _lib.load().vsz_bm3d_workspace_size(C.byref(p))
     if n == 0:
