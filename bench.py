@@ -645,7 +645,9 @@ def main():
         "config": {"workload": "BM3Dv2 two-pass (basic + Wiener ref=basic), chroma, 1920x1080 YUV444PS",
                    "sigma": SIGMA, "path": "plain FP32", **{k: v for k, v in PARAMS.items()},
                    "psnr_y_db": round(psnr_y, 3), "psnr_yuv_db": [round(v, 3) for v in psnr_planes],
-                   "match_ms_per_step": round(t_match * 1e3, 4), "basic_pass_ms": round(t_basic * 1e3, 4)},
+                   "match_ms_per_step": round(t_match * 1e3, 4), "basic_pass_ms": round(t_basic * 1e3, 4),
+                   "matcher": device.match_plan(kp_b),
+                   "filter_tiles": {"basic": list(device.filter_plan(kp_b)), "wiener": list(device.filter_plan(kp_f))}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "kernel": "k_match_lds4 (SSD block matching, both passes)",
