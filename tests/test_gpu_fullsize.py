@@ -5,7 +5,10 @@ lists, distances and outputs.
 
 * 640x360 Gray basic pass, sigma 10, block_step 8, bm_range 9 (the CPU-runnable case);
 * 1920x1080 YUV444 two-pass with chroma, sigma [5, 3, 3], block_step 4, bm_range 16 (the
-  benchmark workload; its matching takes the shared-column kernel).
+  benchmark workload; its matching takes the shared-column kernel, its hard pass three 10 x 17-block
+  filter tiles per CU);
+* 1920x1080 Gray basic pass at block_step 3 and 8: the hard pass's other tile plans (20 x 15
+  blocks three per CU, and 8 x 8 blocks from the two-per-CU planner).
 """
 import numpy as np
 import pytest
@@ -59,3 +62,17 @@ def test_bench_frame_1080p_yuv444_two_pass(orc, torch_gpu):
     psnr = synth.psnr(final[0], clean[0])
     print(f"  bit-identical; PSNR Y {psnr:.3f} dB", flush=True)
     assert psnr > synth.psnr(nz[0], clean[0]) + 3.0
+
+
+@pytest.mark.parametrize("bs,rng,tiles", [(3, 9, (20, 15, 3)), (8, 12, (8, 8, 3))])
+def test_1080p_gray_basic_tile_plans(orc, torch_gpu, bs, rng, tiles):
+    print(f"full size: 1920x1080 Gray basic bs{bs} R{rng}, filter tiles {tiles}", flush=True)
+    W, H = 1920, 1080
+    clean, nz = synth.gray_frame(W, H, 12.0, seed=3 + bs)
+    kp, op = _params(orc, W, H, [12.0] * 3, False, block_step=bs, bm_range=rng, extractor_exp=3)
+    assert device.filter_plan(kp) == tiles
+    src = nz[None, None, None]
+    ores, om, oe = orc.bm3d(op, src)
+    gres, gm = gpu_pass(torch_gpu, kp, src, 1)
+    assert_matches_equal(gm, om, oe)
+    assert np.array_equal(gres, ores)
